@@ -1,0 +1,249 @@
+/*
+ * hhs.h -- C ABI of the MI355X HOLD hot path (libhhs.so).
+ *
+ * Plain C: device pointers, sizes and status codes.  No HIP or torch types
+ * appear here; a stream is passed as `void *` (a hipStream_t, NULL for the
+ * default stream).  Every function returns HHS_OK (0) or a positive status
+ * code and never calls exit(); hhs_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Each entry point names the reference interface it replaces
+ * (dipto4/parallel-hhs, file:line).  include/hhs/hold.hpp re-exposes the
+ * reference's C++ templates (particle_system<T>, hold_step<T,BS>, ...) on
+ * top of these functions.
+ *
+ * Layout: positions, velocities and accelerations are AoS vec3 (x,y,z
+ * interleaved, 3*n values) exactly as vec3<T> arrays in the reference
+ * (src/vec.cuh:5-53, src/particle_system.cuh:23-25); timestep and m are n
+ * values; id and parent_id are n ints.
+ */
+#ifndef HHS_H
+#define HHS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status -- */
+#define HHS_OK 0
+#define HHS_ERR_INVALID 1   /* bad argument (null pointer, negative size)   */
+#define HHS_ERR_ALLOC 2     /* device or host allocation failed             */
+#define HHS_ERR_HIP 3       /* a HIP runtime call or launch failed          */
+#define HHS_ERR_NO_DEVICE 4 /* no usable GPU                                */
+#define HHS_ERR_DEPTH 5     /* HOLD recursion exceeded max_level            */
+#define HHS_ERR_BUDGET 6    /* request exceeds the device-memory budget     */
+#define HHS_ERR_NONFINITE 7 /* NaN/Inf detected in a diagnostic             */
+#define HHS_ERR_WORKSPACE 8 /* workspace too small / not initialised        */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *hhs_last_error(void);
+/* Library version string. */
+const char *hhs_version(void);
+
+/* ------------------------------------------------------- data model -- */
+
+/* Device-side particle set; mirrors the fields of particle_system<T>
+ * (src/particle_system.cuh:20-30).  Any pointer may be NULL when the
+ * called function does not touch that field. */
+typedef struct hhs_particles_f64 {
+    double *pos;      /* 3n */
+    double *vel;      /* 3n */
+    double *acc;      /* 3n */
+    double *timestep; /* n  */
+    double *m;        /* n  */
+    int *id;          /* n  */
+    int *parent_id;   /* n  */
+    int n;
+} hhs_particles_f64;
+
+typedef struct hhs_particles_f32 {
+    float *pos;
+    float *vel;
+    float *acc;
+    float *timestep;
+    float *m;
+    int *id;
+    int *parent_id;
+    int n;
+} hhs_particles_f32;
+
+/* Slow/fast counts; same layout as `struct partition`
+ * (src/particle_system.cuh:8-11). */
+typedef struct hhs_partition {
+    int N_slow;
+    int N_fast;
+} hhs_partition;
+
+/* ---------------------------------------------- device memory helpers -- */
+/* Used by the C++ drop-in header for particle_system<T>::gpu_alloc and
+ * friends (src/particle_system.cuh:135-247). */
+int hhs_device_count(int *count);
+int hhs_malloc(void **ptr, size_t bytes);
+int hhs_free(void *ptr);
+int hhs_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int hhs_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int hhs_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int hhs_memset(void *dst, int value, size_t bytes);
+int hhs_device_synchronize(void);
+int hhs_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------- kicks -- */
+/* Direct-sum kick with Plummer softening eps2 = eps^2 and the attractive
+ * sign (the reference GPU kick is repulsive, src/kick.cuh:23,28; the sign
+ * here follows tests/test_nbforce.cu:14-18).  Pairs whose softened r^2 is
+ * exactly 0 are skipped (src/kick.cuh:25).  acc (if non-NULL) receives the
+ * acceleration of this call; vel += acc * (fac * dt).
+ * FP64 storage: separations are formed in FP64 and rounded to FP32, forces
+ * are summed in FP32 with double-single (two-sum) compensation. */
+
+/* kick_slow (src/kick.cuh:158-168): sinks = sources = sys[0..n). */
+int hhs_kick_self_f64(const hhs_particles_f64 *sys, int n, double dt, double fac,
+                      double eps2, void *stream);
+int hhs_kick_self_f32(const hhs_particles_f32 *sys, int n, float dt, float fac,
+                      float eps2, void *stream);
+
+/* kick_sf (src/kick.cuh:170-181): sinks si[0..n_sink) kicked by sources
+ * so[0..n_src). */
+int hhs_kick_cross_f64(const hhs_particles_f64 *so, const hhs_particles_f64 *si,
+                       int n_sink, int n_src, double dt, double fac, double eps2,
+                       void *stream);
+int hhs_kick_cross_f32(const hhs_particles_f32 *so, const hhs_particles_f32 *si,
+                       int n_sink, int n_src, float dt, float fac, float eps2,
+                       void *stream);
+
+/* Kick tuning: sinks per lane (1,2,4), waves per workgroup sharing one sink
+ * tile (1,2,4,8).  0/0 restores the built-in heuristic.  Applies to calls
+ * issued afterwards from any thread. */
+int hhs_kick_set_config(int sinks_per_lane, int waves_per_group);
+int hhs_kick_get_config(int n_sink, int n_src, int *sinks_per_lane, int *waves_per_group);
+
+/* --------------------------------------------------------- timesteps -- */
+/* timesteps (src/timestep.cuh:103-118): timestep[i] = min over j != i of
+ * the free-fall and fly-by pair timesteps (src/timestep.cuh:17-60), with
+ * r^2 -> r^2 + eps2, start value 1e30.  FP64, same operation order and
+ * IEEE sqrt/divide as the formula, so results are bit-exact against an
+ * FP64 CPU evaluation in that order. */
+int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
+                      void *stream);
+int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,
+                      void *stream);
+
+/* ------------------------------------------------------ scan / split -- */
+/* Device workspace for the single-pass scan (tile counter, epoch, tile
+ * status words).  Allocate hhs_scan_workspace_bytes(n_max) bytes, call
+ * hhs_scan_workspace_init once, then reuse it for any n <= n_max.  Calls
+ * that share one workspace must be ordered (same stream). */
+size_t hhs_scan_workspace_bytes(int n_max);
+int hhs_scan_workspace_init(void *ws, size_t bytes, void *stream);
+
+/* scan_predicate_{small,large,large_ev} (src/split.cuh:437-565):
+ * exclusive prefix sum of n ints, in one pass (decoupled look-back). */
+int hhs_scan_exclusive_i32(const int *in, int *out, int n, void *ws, size_t ws_bytes,
+                           void *stream);
+
+/* split (src/split.cuh:573-610): predicate timestep[i] <= pivot marks a
+ * particle fast; fast ones go to fast[scan[i]], the others to
+ * slow[i - scan[i]] (stable), copying pos, vel, m, timestep, id and
+ * setting parent_id = i.  A lone fast particle stays slow
+ * (src/split.cuh:321-324).  predicate / scanned (n ints each) are written
+ * when non-NULL.  part_dev (device memory) receives the counts; the call
+ * is asynchronous (no host synchronisation). */
+int hhs_split_f64(const hhs_particles_f64 *total, hhs_particles_f64 *slow,
+                  hhs_particles_f64 *fast, int *predicate, int *scanned, double pivot,
+                  int n, hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream);
+int hhs_split_f32(const hhs_particles_f32 *total, hhs_particles_f32 *slow,
+                  hhs_particles_f32 *fast, int *predicate, int *scanned, float pivot,
+                  int n, hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream);
+
+/* Scalar-array split of the reference's scan tests
+ * (src/scan_optimized_3.cuh:485-515, split.cuh naming): values <= pivot go
+ * to fast, the rest to slow, both stable; no lone-particle rule. */
+int hhs_split_values_f32(const float *values, float *slow, float *fast, float pivot,
+                         int n, hhs_partition *part_dev, void *ws, size_t ws_bytes,
+                         void *stream);
+int hhs_split_values_f64(const double *values, double *slow, double *fast,
+                         double pivot, int n, hhs_partition *part_dev, void *ws,
+                         size_t ws_bytes, void *stream);
+
+/* ---------------------------------------------------- drift / update -- */
+/* drift (src/drift.cuh:30-43): pos += vel * (dt * fac), FP64 bit-exact. */
+int hhs_drift_f64(double *pos, const double *vel, int n, double dt, double fac,
+                  void *stream);
+int hhs_drift_f32(float *pos, const float *vel, int n, float dt, float fac,
+                  void *stream);
+
+/* update_system (src/update_system.cuh:29-43):
+ * prev[cur.parent_id[i]] <- cur[i] for pos, vel, m, timestep. */
+int hhs_update_f64(hhs_particles_f64 *prev, const hhs_particles_f64 *cur, int n,
+                   void *stream);
+int hhs_update_f32(hhs_particles_f32 *prev, const hhs_particles_f32 *cur, int n,
+                   void *stream);
+
+/* ---------------------------------------------------------- energy -- */
+/* FP64 kinetic and potential energy, E = sum 1/2 m v^2 -
+ * sum_{i<j} m_i m_j / sqrt(r^2 + eps2) (no reference counterpart; the
+ * diagnostic required for |dE/E|).  Deterministic for a given n.
+ * Blocks the calling thread until the result is on the host. */
+int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n,
+                   double eps2, double *kinetic, double *potential, void *stream);
+
+/* ------------------------------------------------------------- HOLD -- */
+/* nbodysystem_globals<T> (src/particle_system.cuh:65-104) plus the
+ * per-level buffers of nbodysystem_buffers<T> (:107-132), allocated once
+ * and reused (the reference allocates them on every hold_step call and
+ * never frees them, src/hold.cuh:56). */
+typedef struct hhs_globals hhs_globals;
+
+int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps,
+                       int max_level);
+int hhs_globals_destroy(hhs_globals *g);
+/* Scratch exposed as in the reference (device pointers, n_total ints). */
+int *hhs_globals_predicate(hhs_globals *g);
+int *hhs_globals_scanned_predicate(hhs_globals *g);
+double hhs_globals_simtime(const hhs_globals *g);
+void hhs_globals_set_simtime(hhs_globals *g, double t);
+double hhs_globals_eta(const hhs_globals *g);
+double hhs_globals_eps(const hhs_globals *g);
+
+typedef struct hhs_hold_stats {
+    long long calls;        /* hold_step invocations                       */
+    long long interactions; /* kick pair evaluations                        */
+    long long ts_pairs;     /* timestep pair evaluations (ordered)          */
+    int max_level;          /* deepest level reached                        */
+    long long calls_per_level[64];
+} hhs_hold_stats;
+int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out);
+int hhs_globals_reset_stats(hhs_globals *g);
+
+/* hold_step<T,BLOCKSIZE> (src/hold.cuh:51-181) on total[0..n), at
+ * recursion level clevel.  Levels deeper than max_level return
+ * HHS_ERR_DEPTH (the reference has no guard).  Synchronises the stream
+ * once per call to read the slow/fast counts, as the reference does
+ * (src/split.cuh:430). */
+int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int clevel,
+                      double stime, double etime, double dt, int calc_timestep,
+                      void *stream);
+
+/* hold<T> (declared, never defined: src/hold.cuh:20-35): root steps of
+ * dt_root (timesteps recomputed at each root step) until t_end. */
+int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
+                 void *stream);
+
+/* ------------------------------------------- initial conditions (host) -- */
+/* Host-only (no GPU use).  Plummer sphere in Henon units (G = M = 1,
+ * E = -1/4; Aarseth, Henon & Wielen 1974), SplitMix64 stream `seed`,
+ * radii beyond 10 scale lengths rejected, centre of mass at rest at the
+ * origin.  mass_model 0: equal masses 1/n; 1: Salpeter (slope -2.35,
+ * m in [0.1, 100], normalised to total mass 1).  virialize != 0 rescales
+ * velocities to virial ratio 1/2 with the exact O(n^2) potential (eps2).
+ * pos, vel: 3n doubles; m: n doubles. */
+int hhs_ic_plummer(int n, uint64_t seed, int mass_model, int virialize, double eps2,
+                   double *pos, double *vel, double *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HHS_H */

@@ -1,0 +1,100 @@
+// Error reporting, version and device-memory helpers of the C ABI.
+#include <cstdarg>
+#include <cstdio>
+
+#include "hhs_internal.h"
+
+namespace hhs {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+int hip_status(hipError_t e, const char *what, const char *file, int line) {
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return HHS_ERR_NO_DEVICE;
+    if (e == hipErrorOutOfMemory) return HHS_ERR_ALLOC;
+    return HHS_ERR_HIP;
+}
+
+}  // namespace hhs
+
+using namespace hhs;
+
+extern "C" {
+
+const char *hhs_last_error(void) { return g_err; }
+
+const char *hhs_version(void) { return "hhs-mi355x 0.1.0 (gfx950)"; }
+
+int hhs_device_count(int *count) {
+    HHS_REQUIRE(count, "hhs_device_count: null output");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_status(e, "hipGetDeviceCount", __FILE__, __LINE__);
+    }
+    *count = c;
+    return HHS_OK;
+}
+
+int hhs_malloc(void **ptr, size_t bytes) {
+    HHS_REQUIRE(ptr, "hhs_malloc: null output");
+    *ptr = nullptr;
+    if (bytes == 0) return HHS_OK;
+    HHS_HIP(hipMalloc(ptr, bytes));
+    return HHS_OK;
+}
+
+int hhs_free(void *ptr) {
+    if (ptr) HHS_HIP(hipFree(ptr));
+    return HHS_OK;
+}
+
+int hhs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return HHS_OK;
+    HHS_REQUIRE(dst && src, "hhs_memcpy_h2d: null pointer");
+    HHS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return HHS_OK;
+}
+
+int hhs_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return HHS_OK;
+    HHS_REQUIRE(dst && src, "hhs_memcpy_d2h: null pointer");
+    HHS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return HHS_OK;
+}
+
+int hhs_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return HHS_OK;
+    HHS_REQUIRE(dst && src, "hhs_memcpy_d2d: null pointer");
+    HHS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return HHS_OK;
+}
+
+int hhs_memset(void *dst, int value, size_t bytes) {
+    if (bytes == 0) return HHS_OK;
+    HHS_REQUIRE(dst, "hhs_memset: null pointer");
+    HHS_HIP(hipMemset(dst, value, bytes));
+    return HHS_OK;
+}
+
+int hhs_device_synchronize(void) {
+    HHS_HIP(hipDeviceSynchronize());
+    return HHS_OK;
+}
+
+int hhs_stream_synchronize(void *stream) {
+    HHS_HIP(hipStreamSynchronize(as_stream(stream)));
+    return HHS_OK;
+}
+
+}  // extern "C"

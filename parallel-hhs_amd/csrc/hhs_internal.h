@@ -1,0 +1,105 @@
+// Internal declarations shared by the libhhs translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "hhs.h"
+
+namespace hhs {
+
+// Thread-local last-error message (hhs_last_error()).
+void set_error(const char *fmt, ...);
+void clear_error();
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Returns HHS_ERR_HIP (and records the message) when `e` is not hipSuccess.
+int hip_status(hipError_t e, const char *what, const char *file, int line);
+
+#define HHS_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t e__ = (call);                                              \
+        if (e__ != hipSuccess)                                                \
+            return ::hhs::hip_status(e__, #call, __FILE__, __LINE__);         \
+    } while (0)
+
+// Checks the launch that was just issued.
+#define HHS_LAUNCH_CHECK(name)                                                \
+    do {                                                                      \
+        hipError_t e__ = hipGetLastError();                                   \
+        if (e__ != hipSuccess)                                                \
+            return ::hhs::hip_status(e__, name, __FILE__, __LINE__);          \
+    } while (0)
+
+#define HHS_REQUIRE(cond, ...)                                                \
+    do {                                                                      \
+        if (!(cond)) {                                                        \
+            ::hhs::set_error(__VA_ARGS__);                                    \
+            return HHS_ERR_INVALID;                                           \
+        }                                                                     \
+    } while (0)
+
+// ------------------------------------------------------------ particle sets
+template <typename T> struct Particles {
+    T *pos, *vel, *acc, *timestep, *m;
+    int *id, *parent_id;
+    int n;
+};
+
+inline Particles<double> from_c(const hhs_particles_f64 &p) {
+    return {p.pos, p.vel, p.acc, p.timestep, p.m, p.id, p.parent_id, p.n};
+}
+inline Particles<float> from_c(const hhs_particles_f32 &p) {
+    return {p.pos, p.vel, p.acc, p.timestep, p.m, p.id, p.parent_id, p.n};
+}
+
+// ------------------------------------------------------------------- kick
+// Sinks [0, n_sink) of `si` kicked by up to two source segments.  Source
+// segment k is (pos, m, n); a segment with n == 0 is skipped.
+template <typename T> struct KickArgs {
+    const T *si_pos;
+    T *si_vel;
+    T *si_acc;    // may be null
+    int n_sink;
+    const T *s0_pos, *s0_m;
+    int n0;
+    const T *s1_pos, *s1_m;
+    int n1;
+    T h;          // velocity update factor: vel += acc * h   (h = fac * dt)
+    float eps2;
+};
+
+struct KickConfig {
+    int sinks_per_lane;  // 1, 2, 4
+    int waves;           // 1, 2, 4, 8
+};
+
+KickConfig kick_pick_config(int n_sink, int n_src);
+template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t s);
+
+// -------------------------------------------------------------- timesteps
+template <typename T>
+int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
+                     hipStream_t s);
+
+// ----------------------------------------------------------- scan / split
+// Workspace layout: u32 header[4] = {tile counter, epoch, raw fast count, 0}
+// followed by one u64 status word per tile.
+constexpr size_t kScanHeaderBytes = 64;
+size_t scan_ws_bytes(int n_max);
+int scan_ws_init(void *ws, size_t bytes, hipStream_t s);
+
+template <typename T>
+int launch_split(const Particles<T> &total, const Particles<T> &slow, const Particles<T> &fast,
+                 int *predicate, int *scanned, T pivot, int n, hhs_partition *part_dev,
+                 void *ws, size_t ws_bytes, hipStream_t s);
+
+// ------------------------------------------------------------ drift/update
+template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStream_t s);
+template <typename T>
+int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
+
+}  // namespace hhs

@@ -1,0 +1,262 @@
+// HOLD recursion driver (host side) over the gfx950 kernels.
+//
+// Reference: hold_step (src/hold.cuh:51-181), nbodysystem_globals /
+// nbodysystem_buffers (src/particle_system.cuh:65-132); hold() is declared
+// but never defined there (src/hold.cuh:20-35) and is defined here.
+//
+// Differences from the reference, by design:
+//  * per-level slow/fast buffers are allocated once per level (on first
+//    use) and reused; the reference allocates two full particle systems on
+//    every call and never frees them (src/hold.cuh:56);
+//  * the S<-S and S<-F kicks of the slow particles are one launch over two
+//    source segments (slow, fast), so each slow particle's force is summed
+//    once (src/hold.cuh:137-145);
+//  * the only host synchronisation per call is the 8-byte count read after
+//    the split (the reference blocks at the same point, src/split.cuh:430);
+//  * a max_level guard returns HHS_ERR_DEPTH instead of recursing without
+//    bound when timesteps collapse (two coincident particles with eps = 0).
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "hhs_internal.h"
+
+namespace hhs {
+
+struct LevelBuffers {
+    void *block = nullptr;  // one allocation holding both sets
+    Particles<double> slow{}, fast{};
+};
+
+}  // namespace hhs
+
+struct hhs_globals {
+    int n_total = 0;
+    double eta = 0.0, eps = 0.0, eps2 = 0.0;
+    int max_level = 0;
+    int *predicate = nullptr;
+    int *scanned = nullptr;
+    void *scan_ws = nullptr;
+    size_t scan_ws_bytes = 0;
+    hhs_partition *part_dev = nullptr;   // one slot per level
+    hhs_partition *part_host = nullptr;  // pinned mirror
+    std::vector<hhs::LevelBuffers> levels;
+    double simtime = 0.0;
+    hhs_hold_stats stats{};
+    bool ws_ready = false;
+};
+
+namespace hhs {
+namespace {
+
+// Carves one particle set of capacity n out of `p` (96 B per particle).
+char *carve(char *p, int n, Particles<double> &out) {
+    auto take = [&](size_t bytes) {
+        char *r = p;
+        p += (bytes + 255) & ~static_cast<size_t>(255);
+        return r;
+    };
+    const size_t nn = static_cast<size_t>(n);
+    out.pos = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
+    out.vel = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
+    out.acc = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
+    out.timestep = reinterpret_cast<double *>(take(nn * sizeof(double)));
+    out.m = reinterpret_cast<double *>(take(nn * sizeof(double)));
+    out.id = reinterpret_cast<int *>(take(nn * sizeof(int)));
+    out.parent_id = reinterpret_cast<int *>(take(nn * sizeof(int)));
+    out.n = n;
+    return p;
+}
+
+size_t set_bytes(int n) {
+    const size_t nn = static_cast<size_t>(n);
+    auto r = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    return 3 * r(3 * nn * 8) + 2 * r(nn * 8) + 2 * r(nn * 4);
+}
+
+// The level table is sized once (max_level + 1 entries) at creation and
+// never resized, so level descriptors stay valid across the recursion.
+int ensure_level(hhs_globals *g, int level) {
+    if (level >= static_cast<int>(g->levels.size())) {
+        set_error("hold_step: level %d outside the level table", level);
+        return HHS_ERR_DEPTH;
+    }
+    if (g->levels[level].block) return HHS_OK;
+    LevelBuffers &lb = g->levels[level];
+    const size_t bytes = 2 * set_bytes(g->n_total);
+    HHS_HIP(hipMalloc(&lb.block, bytes));
+    char *p = static_cast<char *>(lb.block);
+    p = carve(p, g->n_total, lb.slow);
+    carve(p, g->n_total, lb.fast);
+    return HHS_OK;
+}
+
+int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
+             bool calc, hipStream_t st) {
+    if (clevel > g->max_level) {
+        set_error("hold_step: recursion level %d exceeds max_level %d (timesteps collapsed?)",
+                  clevel, g->max_level);
+        return HHS_ERR_DEPTH;
+    }
+    hhs_hold_stats &S = g->stats;
+    S.calls++;
+    if (clevel < 64) S.calls_per_level[clevel]++;
+    if (clevel > S.max_level) S.max_level = clevel;
+
+    int rc;
+    // src/hold.cuh:74-76
+    if (calc) {
+        rc = launch_timesteps<double>(total.pos, total.vel, total.m, total.timestep, n, g->eta,
+                                      g->eps2, st);
+        if (rc) return rc;
+        S.ts_pairs += static_cast<long long>(n) * (n - 1);
+    }
+    if ((rc = ensure_level(g, clevel))) return rc;
+    const LevelBuffers lb = g->levels[clevel];  // by value: plain pointers
+    // src/hold.cuh:79-83, then the count read-back (src/split.cuh:430)
+    rc = launch_split<double>(total, lb.slow, lb.fast, nullptr, nullptr, dt, n,
+                              g->part_dev + clevel, g->scan_ws, g->scan_ws_bytes, st);
+    if (rc) return rc;
+    HHS_HIP(hipMemcpyAsync(g->part_host + clevel, g->part_dev + clevel, sizeof(hhs_partition),
+                           hipMemcpyDeviceToHost, st));
+    HHS_HIP(hipStreamSynchronize(st));
+    const int ns = g->part_host[clevel].N_slow;
+    const int nf = g->part_host[clevel].N_fast;
+
+    // src/hold.cuh:106-114
+    if (nf == 0) g->simtime += dt;
+    // src/hold.cuh:116-121
+    if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, false, st))) return rc;
+    if (ns > 0) {
+        // src/hold.cuh:123-124
+        if ((rc = launch_drift<double>(lb.slow.pos, lb.slow.vel, ns, dt * 0.5, st))) return rc;
+        // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow
+        KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
+                            lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
+                            lb.fast.m,   nf,         dt,          static_cast<float>(g->eps2)};
+        if ((rc = launch_kick<double>(ks, st))) return rc;
+        S.interactions += static_cast<long long>(ns) * (ns + nf);
+        if (nf > 0) {
+            KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
+                                lb.slow.m,   ns,          nullptr,     nullptr, 0,
+                                dt,          static_cast<float>(g->eps2)};
+            if ((rc = launch_kick<double>(kf, st))) return rc;
+            S.interactions += static_cast<long long>(nf) * ns;
+        }
+        // src/hold.cuh:158-159
+        if ((rc = launch_drift<double>(lb.slow.pos, lb.slow.vel, ns, dt * 0.5, st))) return rc;
+    }
+    // src/hold.cuh:168-173
+    if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, true, st))) return rc;
+    // src/hold.cuh:177-178 (the deeper levels reuse lb, so scatter back now)
+    if ((rc = launch_update<double>(total, lb.fast, nf, st))) return rc;
+    if ((rc = launch_update<double>(total, lb.slow, ns, st))) return rc;
+    return HHS_OK;
+}
+
+}  // namespace
+}  // namespace hhs
+
+using namespace hhs;
+
+extern "C" {
+
+int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, int max_level) {
+    HHS_REQUIRE(out, "globals_create: null output");
+    *out = nullptr;
+    HHS_REQUIRE(n_total > 0, "globals_create: n_total must be positive");
+    HHS_REQUIRE(eta > 0.0, "globals_create: eta must be positive");
+    HHS_REQUIRE(eps >= 0.0, "globals_create: eps must be >= 0");
+    if (max_level <= 0) max_level = 48;
+    HHS_REQUIRE(max_level < 64, "globals_create: max_level must be < 64");
+    hhs_globals *g = new (std::nothrow) hhs_globals();
+    if (!g) {
+        set_error("globals_create: out of host memory");
+        return HHS_ERR_ALLOC;
+    }
+    g->n_total = n_total;
+    g->eta = eta;
+    g->eps = eps;
+    g->eps2 = eps * eps;
+    g->max_level = max_level;
+    g->levels.resize(static_cast<size_t>(max_level) + 1);
+    g->scan_ws_bytes = scan_ws_bytes(n_total);
+    hipError_t e = hipMalloc(&g->predicate, sizeof(int) * static_cast<size_t>(n_total));
+    if (e == hipSuccess) e = hipMalloc(&g->scanned, sizeof(int) * static_cast<size_t>(n_total));
+    if (e == hipSuccess) e = hipMalloc(&g->scan_ws, g->scan_ws_bytes);
+    if (e == hipSuccess) e = hipMalloc(&g->part_dev, sizeof(hhs_partition) * (max_level + 1));
+    if (e == hipSuccess)
+        e = hipHostMalloc(&g->part_host, sizeof(hhs_partition) * (max_level + 1),
+                          hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemset(g->scan_ws, 0, g->scan_ws_bytes);
+    if (e != hipSuccess) {
+        int rc = hip_status(e, "globals_create", __FILE__, __LINE__);
+        hhs_globals_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return HHS_OK;
+}
+
+int hhs_globals_destroy(hhs_globals *g) {
+    if (!g) return HHS_OK;
+    for (auto &lb : g->levels)
+        if (lb.block) (void)hipFree(lb.block);
+    if (g->predicate) (void)hipFree(g->predicate);
+    if (g->scanned) (void)hipFree(g->scanned);
+    if (g->scan_ws) (void)hipFree(g->scan_ws);
+    if (g->part_dev) (void)hipFree(g->part_dev);
+    if (g->part_host) (void)hipHostFree(g->part_host);
+    delete g;
+    return HHS_OK;
+}
+
+int *hhs_globals_predicate(hhs_globals *g) { return g ? g->predicate : nullptr; }
+int *hhs_globals_scanned_predicate(hhs_globals *g) { return g ? g->scanned : nullptr; }
+double hhs_globals_simtime(const hhs_globals *g) { return g ? g->simtime : 0.0; }
+void hhs_globals_set_simtime(hhs_globals *g, double t) {
+    if (g) g->simtime = t;
+}
+double hhs_globals_eta(const hhs_globals *g) { return g ? g->eta : 0.0; }
+double hhs_globals_eps(const hhs_globals *g) { return g ? g->eps : 0.0; }
+
+int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out) {
+    HHS_REQUIRE(g && out, "globals_stats: null argument");
+    *out = g->stats;
+    return HHS_OK;
+}
+
+int hhs_globals_reset_stats(hhs_globals *g) {
+    HHS_REQUIRE(g, "globals_reset_stats: null globals");
+    std::memset(&g->stats, 0, sizeof(g->stats));
+    return HHS_OK;
+}
+
+int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int clevel,
+                      double stime, double etime, double dt, int calc_timestep, void *stream) {
+    (void)stime;
+    (void)etime;  // the system is autonomous; the reference never reads them
+    HHS_REQUIRE(g && total, "hold_step: null argument");
+    HHS_REQUIRE(n > 0 && n <= total->n && n <= g->n_total,
+                "hold_step: n=%d outside [1, min(%d, %d)]", n, total->n, g->n_total);
+    HHS_REQUIRE(clevel >= 0, "hold_step: negative level");
+    HHS_REQUIRE(dt > 0.0, "hold_step: dt must be positive");
+    HHS_REQUIRE(total->pos && total->vel && total->m && total->timestep,
+                "hold_step: null field in total");
+    return hold_rec(g, from_c(*total), n, clevel, dt, calc_timestep != 0, as_stream(stream));
+}
+
+int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
+                 void *stream) {
+    HHS_REQUIRE(g && sys, "hold: null argument");
+    HHS_REQUIRE(dt_root > 0.0, "hold: dt_root must be positive");
+    double t = 0.0;
+    while (t < t_end) {
+        int rc = hhs_hold_step_f64(g, sys, sys->n, 0, t, t + dt_root, dt_root, 1, stream);
+        if (rc) return rc;
+        t += dt_root;
+    }
+    return HHS_OK;
+}
+
+}  // extern "C"

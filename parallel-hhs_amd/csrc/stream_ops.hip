@@ -1,0 +1,189 @@
+// Streaming kernels: drift, parent scatter-back, FP64 energy.
+//
+// Reference: _drift (src/drift.cuh:16-43), _update_system
+// (src/update_system.cuh:10-43).  The energy diagnostic has no reference
+// counterpart.  Drift and update are HBM-bound (72 B and 132 B per
+// particle); they run without FMA contraction so FP64 results are bit-exact
+// against the same expression on the CPU.
+#include <vector>
+
+#include "hhs_internal.h"
+
+namespace hhs {
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void drift_kernel(T *__restrict__ pos,
+                                                    const T *__restrict__ vel, int n3, T h) {
+#pragma clang fp contract(off)
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += stride)
+        pos[k] = pos[k] + vel[k] * h;  // src/drift.cuh:25
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void update_kernel(Particles<T> prev, Particles<T> cur,
+                                                     int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = cur.parent_id[i];  // src/update_system.cuh:14-22
+    prev.pos[3 * p + 0] = cur.pos[3 * i + 0];
+    prev.pos[3 * p + 1] = cur.pos[3 * i + 1];
+    prev.pos[3 * p + 2] = cur.pos[3 * i + 2];
+    prev.vel[3 * p + 0] = cur.vel[3 * i + 0];
+    prev.vel[3 * p + 1] = cur.vel[3 * i + 1];
+    prev.vel[3 * p + 2] = cur.vel[3 * i + 2];
+    prev.m[p] = cur.m[i];
+    prev.timestep[p] = cur.timestep[i];
+}
+
+// Row sums of the potential: row[i] = m_i * sum_{j>i} m_j / sqrt(r^2+eps2),
+// j ascending (the order of a serial CPU loop), plus the kinetic term of i.
+// The j loop is wave-uniform (scalar loads); lanes with j <= i skip.
+__global__ __launch_bounds__(256) void energy_rows_kernel(const double *__restrict__ pos,
+                                                          const double *__restrict__ vel,
+                                                          const double *__restrict__ m, int n,
+                                                          double eps2, double *__restrict__ row,
+                                                          double *__restrict__ kin) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ic = i < n ? i : n - 1;
+    const double xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
+    const int wave_first = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x +
+                                                          (threadIdx.x & ~63));
+    double s = 0.0;
+    for (int j = wave_first + 1; j < n; ++j) {
+        const double dx = pos[3 * j] - xi;
+        const double dy = pos[3 * j + 1] - yi;
+        const double dz = pos[3 * j + 2] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double t = m[j] / sqrt(r2);
+        if (j > i && r2 != 0.0) s += t;
+    }
+    if (i < n) {
+        row[i] = m[i] * s;
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        kin[i] = 0.5 * m[i] * (vx * vx + vy * vy + vz * vz);
+    }
+}
+
+// Fixed-order reduction of row[] and kin[] in one workgroup: lane t sums
+// indices t, t+1024, ... serially, then a fixed tree combines the lanes.
+__global__ __launch_bounds__(1024) void energy_reduce_kernel(const double *__restrict__ row,
+                                                             const double *__restrict__ kin,
+                                                             int n, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double sp[1024], sk[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int i = t; i < n; i += 1024) {
+        a += row[i];
+        b += kin[i];
+    }
+    sp[t] = a;
+    sk[t] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            sp[t] += sp[t + w];
+            sk[t] += sk[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = sk[0];
+        out[1] = -sp[0];
+    }
+}
+
+}  // namespace
+
+template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStream_t s) {
+    HHS_REQUIRE(n >= 0, "drift: negative n");
+    if (n == 0) return HHS_OK;
+    HHS_REQUIRE(pos && vel, "drift: null array");
+    const int n3 = 3 * n;
+    int blocks = (n3 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL((drift_kernel<T>), dim3(blocks), dim3(256), 0, s, pos, vel, n3, h);
+    HHS_LAUNCH_CHECK("drift_kernel");
+    return HHS_OK;
+}
+
+template <typename T>
+int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s) {
+    HHS_REQUIRE(n >= 0, "update: negative n");
+    if (n == 0) return HHS_OK;  // src/update_system.cuh:32
+    HHS_REQUIRE(prev.pos && prev.vel && prev.m && prev.timestep, "update: null field in prev");
+    HHS_REQUIRE(cur.pos && cur.vel && cur.m && cur.timestep && cur.parent_id,
+                "update: null field in cur");
+    hipLaunchKernelGGL((update_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s, prev, cur, n);
+    HHS_LAUNCH_CHECK("update_kernel");
+    return HHS_OK;
+}
+
+template int launch_drift<double>(double *, const double *, int, double, hipStream_t);
+template int launch_drift<float>(float *, const float *, int, float, hipStream_t);
+template int launch_update<double>(const Particles<double> &, const Particles<double> &, int,
+                                   hipStream_t);
+template int launch_update<float>(const Particles<float> &, const Particles<float> &, int,
+                                  hipStream_t);
+
+}  // namespace hhs
+
+using namespace hhs;
+
+extern "C" {
+
+int hhs_drift_f64(double *pos, const double *vel, int n, double dt, double fac, void *stream) {
+    return launch_drift<double>(pos, vel, n, dt * fac, as_stream(stream));
+}
+
+int hhs_drift_f32(float *pos, const float *vel, int n, float dt, float fac, void *stream) {
+    return launch_drift<float>(pos, vel, n, dt * fac, as_stream(stream));
+}
+
+int hhs_update_f64(hhs_particles_f64 *prev, const hhs_particles_f64 *cur, int n, void *stream) {
+    HHS_REQUIRE(prev && cur, "update: null system");
+    HHS_REQUIRE(n >= 0 && n <= cur->n, "update: n=%d outside [0, %d]", n, cur->n);
+    return launch_update<double>(from_c(*prev), from_c(*cur), n, as_stream(stream));
+}
+
+int hhs_update_f32(hhs_particles_f32 *prev, const hhs_particles_f32 *cur, int n, void *stream) {
+    HHS_REQUIRE(prev && cur, "update: null system");
+    HHS_REQUIRE(n >= 0 && n <= cur->n, "update: n=%d outside [0, %d]", n, cur->n);
+    return launch_update<float>(from_c(*prev), from_c(*cur), n, as_stream(stream));
+}
+
+int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n, double eps2,
+                   double *kinetic, double *potential, void *stream) {
+    HHS_REQUIRE(kinetic && potential, "energy: null output");
+    HHS_REQUIRE(n >= 0, "energy: negative n");
+    *kinetic = 0.0;
+    *potential = 0.0;
+    if (n == 0) return HHS_OK;
+    HHS_REQUIRE(pos && vel && m, "energy: null array");
+    hipStream_t st = as_stream(stream);
+    double *buf = nullptr;
+    HHS_HIP(hipMalloc(&buf, (2 * static_cast<size_t>(n) + 2) * sizeof(double)));
+    double *row = buf, *kin = buf + n, *out = buf + 2 * static_cast<size_t>(n);
+    int rc = HHS_OK;
+    hipLaunchKernelGGL(energy_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, m, n,
+                       eps2, row, kin);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(1024), 0, st, row, kin, n, out);
+        e = hipGetLastError();
+    }
+    double host[2] = {0.0, 0.0};
+    if (e == hipSuccess) e = hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_status(e, "energy", __FILE__, __LINE__);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    *kinetic = host[0];
+    *potential = host[1];
+    return HHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,368 @@
+"""ctypes binding of libhhs.so (the C ABI declared in include/hhs.h).
+
+Device arrays are passed as torch tensors (PyTorch only supplies device
+memory and streams); every call goes straight to the C ABI.  There is no
+CPU fallback: if the library is not built, importing this module raises.
+
+Reference API mirrored (dipto4/parallel-hhs): particle_system<T>
+(src/particle_system.cuh:20-54), split (src/split.cuh:573-610), kick_slow /
+kick_sf (src/kick.cuh:158-181), timesteps (src/timestep.cuh:103-118), drift
+(src/drift.cuh:30-43), update_system (src/update_system.cuh:29-43) and
+hold_step / hold (src/hold.cuh:17-23,51-181).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhhs.so")
+
+HHS_OK = 0
+HHS_ERR_INVALID = 1
+HHS_ERR_ALLOC = 2
+HHS_ERR_HIP = 3
+HHS_ERR_NO_DEVICE = 4
+HHS_ERR_DEPTH = 5
+HHS_ERR_BUDGET = 6
+HHS_ERR_NONFINITE = 7
+HHS_ERR_WORKSPACE = 8
+
+
+class HHSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hhs error {code}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `make -C parallel-hhs_amd` "
+        "(or __graft_entry__.build()) first")
+
+# PyTorch-ROCm bundles its own HIP runtime with the same soname
+# (libamdhip64.so.7) as /opt/rocm's.  Loading torch first makes libhhs bind
+# to the runtime already in the process; loading libhhs first would bring in
+# a second HIP/HSA runtime next to torch's and crash on the first device call.
+try:
+    import torch  # noqa: F401
+except ImportError:  # C++/ctypes-only use without PyTorch
+    torch = None
+
+_lib = C.CDLL(LIB_PATH)
+
+
+class ParticlesF64(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("vel", C.c_void_p), ("acc", C.c_void_p),
+                ("timestep", C.c_void_p), ("m", C.c_void_p), ("id", C.c_void_p),
+                ("parent_id", C.c_void_p), ("n", C.c_int)]
+
+
+ParticlesF32 = type("ParticlesF32", (C.Structure,), {"_fields_": ParticlesF64._fields_})
+
+
+class Partition(C.Structure):
+    _fields_ = [("N_slow", C.c_int), ("N_fast", C.c_int)]
+
+
+class HoldStats(C.Structure):
+    _fields_ = [("calls", C.c_longlong), ("interactions", C.c_longlong),
+                ("ts_pairs", C.c_longlong), ("max_level", C.c_int),
+                ("calls_per_level", C.c_longlong * 64)]
+
+
+_vp, _i, _d, _f, _sz = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
+_P64, _P32 = C.POINTER(ParticlesF64), C.POINTER(ParticlesF32)
+
+_SIGS = {
+    "hhs_last_error": (C.c_char_p, []),
+    "hhs_version": (C.c_char_p, []),
+    "hhs_device_count": (_i, [C.POINTER(_i)]),
+    "hhs_malloc": (_i, [C.POINTER(_vp), _sz]),
+    "hhs_free": (_i, [_vp]),
+    "hhs_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "hhs_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "hhs_memcpy_d2d": (_i, [_vp, _vp, _sz]),
+    "hhs_memset": (_i, [_vp, _i, _sz]),
+    "hhs_device_synchronize": (_i, []),
+    "hhs_stream_synchronize": (_i, [_vp]),
+    "hhs_kick_self_f64": (_i, [_P64, _i, _d, _d, _d, _vp]),
+    "hhs_kick_self_f32": (_i, [_P32, _i, _f, _f, _f, _vp]),
+    "hhs_kick_cross_f64": (_i, [_P64, _P64, _i, _i, _d, _d, _d, _vp]),
+    "hhs_kick_cross_f32": (_i, [_P32, _P32, _i, _i, _f, _f, _f, _vp]),
+    "hhs_kick_set_config": (_i, [_i, _i]),
+    "hhs_kick_get_config": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
+    "hhs_timesteps_f64": (_i, [_P64, _i, _d, _d, _vp]),
+    "hhs_timesteps_f32": (_i, [_P32, _i, _f, _f, _vp]),
+    "hhs_scan_workspace_bytes": (_sz, [_i]),
+    "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
+    "hhs_scan_exclusive_i32": (_i, [_vp, _vp, _i, _vp, _sz, _vp]),
+    "hhs_split_f64": (_i, [_P64, _P64, _P64, _vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
+    "hhs_split_f32": (_i, [_P32, _P32, _P32, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
+    "hhs_split_values_f32": (_i, [_vp, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
+    "hhs_split_values_f64": (_i, [_vp, _vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
+    "hhs_drift_f64": (_i, [_vp, _vp, _i, _d, _d, _vp]),
+    "hhs_drift_f32": (_i, [_vp, _vp, _i, _f, _f, _vp]),
+    "hhs_update_f64": (_i, [_P64, _P64, _i, _vp]),
+    "hhs_update_f32": (_i, [_P32, _P32, _i, _vp]),
+    "hhs_energy_f64": (_i, [_vp, _vp, _vp, _i, _d, C.POINTER(_d), C.POINTER(_d), _vp]),
+    "hhs_globals_create": (_i, [C.POINTER(_vp), _i, _d, _d, _i]),
+    "hhs_globals_destroy": (_i, [_vp]),
+    "hhs_globals_predicate": (_vp, [_vp]),
+    "hhs_globals_scanned_predicate": (_vp, [_vp]),
+    "hhs_globals_simtime": (_d, [_vp]),
+    "hhs_globals_set_simtime": (None, [_vp, _d]),
+    "hhs_globals_eta": (_d, [_vp]),
+    "hhs_globals_eps": (_d, [_vp]),
+    "hhs_globals_stats": (_i, [_vp, C.POINTER(HoldStats)]),
+    "hhs_globals_reset_stats": (_i, [_vp]),
+    "hhs_hold_step_f64": (_i, [_vp, _P64, _i, _i, _d, _d, _d, _i, _vp]),
+    "hhs_hold_f64": (_i, [_vp, _P64, _d, _d, _vp]),
+    "hhs_ic_plummer": (_i, [_i, C.c_uint64, _i, _i, _d, _vp, _vp, _vp]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+def last_error() -> str:
+    return _lib.hhs_last_error().decode()
+
+
+def check(rc: int) -> None:
+    if rc != HHS_OK:
+        raise HHSError(rc, last_error())
+
+
+def raw():
+    """The loaded ctypes library (for tests that call entry points directly)."""
+    return _lib
+
+
+# ---------------------------------------------------------------- helpers
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+@dataclass
+class DeviceParticles:
+    """Device particle set backed by torch tensors (one field per
+    particle_system<T> member, AoS vec3 for pos/vel/acc)."""
+    pos: object
+    vel: object
+    acc: object
+    timestep: object
+    m: object
+    id: object
+    parent_id: object
+
+    @property
+    def n(self) -> int:
+        return int(self.m.shape[0])
+
+    @classmethod
+    def empty(cls, n: int, dtype=None, device="cuda"):
+        import torch
+        dtype = dtype or torch.float64
+        return cls(pos=torch.zeros(n, 3, dtype=dtype, device=device),
+                   vel=torch.zeros(n, 3, dtype=dtype, device=device),
+                   acc=torch.zeros(n, 3, dtype=dtype, device=device),
+                   timestep=torch.zeros(n, dtype=dtype, device=device),
+                   m=torch.zeros(n, dtype=dtype, device=device),
+                   id=torch.zeros(n, dtype=torch.int32, device=device),
+                   parent_id=torch.zeros(n, dtype=torch.int32, device=device))
+
+    @classmethod
+    def from_host(cls, pos, vel, m, timestep=None, ids=None, dtype=None, device="cuda"):
+        import numpy as np
+        import torch
+        dtype = dtype or torch.float64
+        n = len(m)
+        p = cls.empty(n, dtype, device)
+        p.pos.copy_(torch.as_tensor(np.asarray(pos).reshape(n, 3)))
+        p.vel.copy_(torch.as_tensor(np.asarray(vel).reshape(n, 3)))
+        p.m.copy_(torch.as_tensor(np.asarray(m)))
+        if timestep is not None:
+            p.timestep.copy_(torch.as_tensor(np.asarray(timestep)))
+        p.id.copy_(torch.as_tensor(np.arange(n, dtype=np.int32) if ids is None
+                                   else np.asarray(ids, dtype=np.int32)))
+        return p
+
+    def struct(self):
+        import torch
+        cls = ParticlesF64 if self.m.dtype == torch.float64 else ParticlesF32
+        return cls(self.pos.data_ptr(), self.vel.data_ptr(), self.acc.data_ptr(),
+                   self.timestep.data_ptr(), self.m.data_ptr(), self.id.data_ptr(),
+                   self.parent_id.data_ptr(), self.n)
+
+    def is_f64(self) -> bool:
+        import torch
+        return self.m.dtype == torch.float64
+
+
+def _sfx(p: DeviceParticles) -> str:
+    return "f64" if p.is_f64() else "f32"
+
+
+def kick_self(sys: DeviceParticles, n=None, dt=1.0, fac=1.0, eps2=0.0, stream=None):
+    n = sys.n if n is None else n
+    s = sys.struct()
+    check(getattr(_lib, "hhs_kick_self_" + _sfx(sys))(C.byref(s), n, dt, fac, eps2, _stream(stream)))
+
+
+def kick_cross(so: DeviceParticles, si: DeviceParticles, n_sink=None, n_src=None, dt=1.0,
+               fac=1.0, eps2=0.0, stream=None):
+    n_sink = si.n if n_sink is None else n_sink
+    n_src = so.n if n_src is None else n_src
+    a, b = so.struct(), si.struct()
+    check(getattr(_lib, "hhs_kick_cross_" + _sfx(si))(C.byref(a), C.byref(b), n_sink, n_src, dt,
+                                                      fac, eps2, _stream(stream)))
+
+
+def kick_set_config(sinks_per_lane: int, waves: int):
+    check(_lib.hhs_kick_set_config(sinks_per_lane, waves))
+
+
+def kick_get_config(n_sink: int, n_src: int):
+    s, w = C.c_int(), C.c_int()
+    check(_lib.hhs_kick_get_config(n_sink, n_src, C.byref(s), C.byref(w)))
+    return s.value, w.value
+
+
+def timesteps(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, stream=None):
+    n = sys.n if n is None else n
+    s = sys.struct()
+    check(getattr(_lib, "hhs_timesteps_" + _sfx(sys))(C.byref(s), n, eta, eps2, _stream(stream)))
+
+
+class ScanWorkspace:
+    """Device workspace of the single-pass scan (allocated through torch)."""
+
+    def __init__(self, n_max: int, stream=None):
+        import torch
+        self.nbytes = int(_lib.hhs_scan_workspace_bytes(int(n_max)))
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device="cuda")
+        check(_lib.hhs_scan_workspace_init(_ptr(self.buf), self.nbytes, _stream(stream)))
+
+    @property
+    def ptr(self):
+        return _ptr(self.buf)
+
+
+def scan_exclusive(inp, out, ws: ScanWorkspace, n=None, stream=None):
+    n = int(inp.shape[0]) if n is None else n
+    check(_lib.hhs_scan_exclusive_i32(_ptr(inp), _ptr(out), n, ws.ptr, ws.nbytes, _stream(stream)))
+
+
+def split(total: DeviceParticles, slow: DeviceParticles, fast: DeviceParticles, pivot: float,
+          part_dev, ws: ScanWorkspace, n=None, predicate=None, scanned=None, stream=None):
+    n = total.n if n is None else n
+    t, s, f = total.struct(), slow.struct(), fast.struct()
+    check(getattr(_lib, "hhs_split_" + _sfx(total))(
+        C.byref(t), C.byref(s), C.byref(f), _ptr(predicate), _ptr(scanned), pivot, n,
+        _ptr(part_dev), ws.ptr, ws.nbytes, _stream(stream)))
+
+
+def split_values(values, slow, fast, pivot: float, part_dev, ws: ScanWorkspace, n=None,
+                 stream=None):
+    import torch
+    n = int(values.shape[0]) if n is None else n
+    fn = _lib.hhs_split_values_f64 if values.dtype == torch.float64 else _lib.hhs_split_values_f32
+    check(fn(_ptr(values), _ptr(slow), _ptr(fast), pivot, n, _ptr(part_dev), ws.ptr, ws.nbytes,
+             _stream(stream)))
+
+
+def drift(pos, vel, n=None, dt=1.0, fac=1.0, stream=None):
+    import torch
+    n = int(pos.shape[0]) if n is None else n
+    fn = _lib.hhs_drift_f64 if pos.dtype == torch.float64 else _lib.hhs_drift_f32
+    check(fn(_ptr(pos), _ptr(vel), n, dt, fac, _stream(stream)))
+
+
+def update(prev: DeviceParticles, cur: DeviceParticles, n=None, stream=None):
+    n = cur.n if n is None else n
+    a, b = prev.struct(), cur.struct()
+    check(getattr(_lib, "hhs_update_" + _sfx(cur))(C.byref(a), C.byref(b), n, _stream(stream)))
+
+
+def energy(pos, vel, m, eps2=0.0, n=None, stream=None):
+    n = int(m.shape[0]) if n is None else n
+    ke, pe = C.c_double(), C.c_double()
+    check(_lib.hhs_energy_f64(_ptr(pos), _ptr(vel), _ptr(m), n, eps2, C.byref(ke), C.byref(pe),
+                              _stream(stream)))
+    return ke.value, pe.value
+
+
+class Globals:
+    """nbodysystem_globals<double> + per-level buffers (hhs_globals)."""
+
+    def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48):
+        h = C.c_void_p()
+        check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.hhs_globals_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def simtime(self) -> float:
+        return _lib.hhs_globals_simtime(self.h)
+
+    @simtime.setter
+    def simtime(self, t: float):
+        _lib.hhs_globals_set_simtime(self.h, t)
+
+    def stats(self) -> dict:
+        st = HoldStats()
+        check(_lib.hhs_globals_stats(self.h, C.byref(st)))
+        lv = [int(x) for x in st.calls_per_level[:st.max_level + 1]]
+        return {"calls": st.calls, "interactions": st.interactions, "ts_pairs": st.ts_pairs,
+                "max_level": st.max_level, "calls_per_level": lv}
+
+    def reset_stats(self):
+        check(_lib.hhs_globals_reset_stats(self.h))
+
+    def hold_step(self, sys: DeviceParticles, dt: float, calc_timestep=True, clevel=0, n=None,
+                  stream=None):
+        n = sys.n if n is None else n
+        s = sys.struct()
+        check(_lib.hhs_hold_step_f64(self.h, C.byref(s), n, clevel, 0.0, dt, dt,
+                                     1 if calc_timestep else 0, _stream(stream)))
+
+    def hold(self, sys: DeviceParticles, t_end: float, dt_root: float, stream=None):
+        s = sys.struct()
+        check(_lib.hhs_hold_f64(self.h, C.byref(s), t_end, dt_root, _stream(stream)))
+
+
+def ic_plummer(n: int, seed: int, mass_model: int = 0, virialize: bool = True,
+               eps2: float = 0.0):
+    """Host-side Plummer sphere (numpy arrays pos[n,3], vel[n,3], m[n])."""
+    import numpy as np
+    pos = np.zeros((n, 3), dtype=np.float64)
+    vel = np.zeros((n, 3), dtype=np.float64)
+    m = np.zeros(n, dtype=np.float64)
+    check(_lib.hhs_ic_plummer(n, seed, mass_model, 1 if virialize else 0, eps2,
+                              pos.ctypes.data, vel.ctypes.data, m.ctypes.data))
+    return pos, vel, m

@@ -1,0 +1,99 @@
+"""HOLD recursion on the GPU against the FP64 CPU HOLD oracle.
+
+Tolerances: positions and velocities after one root step agree to 1e-6
+relative to the system's rms values (FP32 forces vs FP64); the recursion
+(calls per level, simulated time) is identical; the relative energy error
+|dE/E| after a fixed integration time agrees with the FP64 HOLD run to
+within 10%, measured at eta = 0.1 where the integrator error dominates the
+FP32 force rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import plummer
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_pair(hhs, oracle, pos, vel, m, eta, eps, t_end, dt_root):
+    eps2 = eps * eps
+    host = oracle.HostParticles(len(m), pos, vel, m)
+    rc, ost = oracle.hold(host, t_end, dt_root, eta, eps2)
+    assert rc == 0
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(len(m), eta, eps)
+    g.hold(d, t_end, dt_root)
+    return host, ost, d, g
+
+
+@pytest.mark.parametrize("n", [2, 16, 128, 512])
+def test_hold_one_root_step_matches_oracle(hhs, oracle, torch_cuda, n):
+    eps, eta = 1e-2, 0.01
+    pos, vel, m = plummer(hhs, n, seed=1000 + n, eps2=eps * eps)
+    host, ost, d, g = _run_pair(hhs, oracle, pos, vel, m, eta, eps, 1.0 / 32, 1.0 / 32)
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert st["calls"] == ost.calls
+    assert st["calls_per_level"] == [int(x) for x in ost.calls_per_level[:ost.max_level + 1]]
+    assert g.simtime == pytest.approx(ost.simtime, rel=0, abs=0)
+    p, v = d.pos.cpu().numpy(), d.vel.cpu().numpy()
+    assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
+    assert np.abs(v - host.vel).max() <= 1e-6 * np.sqrt((host.vel**2).mean())
+    assert np.array_equal(d.m.cpu().numpy(), host.m)
+
+
+def test_hold_energy_error_matches_fp64_run(hhs, oracle, torch_cuda):
+    n, eps, eta = 256, 1e-2, 0.1
+    pos, vel, m = plummer(hhs, n, seed=77, eps2=eps * eps)
+    e0 = oracle.energy(pos, vel, m, eps * eps)
+    host, ost, d, g = _run_pair(hhs, oracle, pos, vel, m, eta, eps, 0.5, 1.0 / 16)
+    e_cpu = oracle.energy(host.pos, host.vel, host.m, eps * eps)
+    ke, pe = hhs.energy(d.pos, d.vel, d.m, eps * eps)
+    de_cpu = abs((e_cpu - e0) / e0)
+    de_gpu = abs((ke + pe - e0) / e0)
+    assert de_cpu > 1e-9, "integrator error too small for a meaningful comparison"
+    assert abs(de_gpu - de_cpu) <= 0.1 * de_cpu, f"|dE/E| gpu {de_gpu:.4e} cpu {de_cpu:.4e}"
+
+
+def test_hold_kepler_period(hhs, torch_cuda):
+    """Eccentric two-body orbit (a = 1, e = 0.5, M = 1): after one period
+    T = 2*pi the bodies return to their start (second-order error)."""
+    e, mtot = 0.5, 1.0
+    rp = 1.0 - e
+    vp = np.sqrt(mtot * (1 + e) / rp)
+    m = np.array([0.5, 0.5])
+    pos = np.array([[-rp / 2, 0, 0], [rp / 2, 0, 0]])
+    vel = np.array([[0, -vp / 2, 0], [0, vp / 2, 0]])
+    T = 2 * np.pi
+    nsteps = 4096
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(2, 0.01, 0.0)
+    for _ in range(nsteps):
+        g.hold_step(d, T / nsteps)
+    torch_cuda.cuda.synchronize()
+    p = d.pos.cpu().numpy()
+    assert np.abs(p - pos).max() < 1e-3
+
+
+def test_hold_momentum_conserved(hhs, torch_cuda):
+    n, eps = 300, 1e-3
+    pos, vel, m = plummer(hhs, n, seed=8, mass_model=1, eps2=eps * eps)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(n, 0.01, eps)
+    g.hold(d, 0.125, 0.125)
+    torch_cuda.cuda.synchronize()
+    p1 = (m[:, None] * d.vel.cpu().numpy()).sum(0)
+    scale = (m[:, None] * np.abs(d.vel.cpu().numpy())).sum(0)
+    assert np.all(np.abs(p1) <= 1e-6 * scale)
+
+
+def test_hold_depth_guard(hhs, torch_cuda):
+    """Two coincident particles with eps = 0 give tau = 0: the reference
+    would recurse forever; here the level guard returns HHS_ERR_DEPTH."""
+    pos, vel, m = plummer(hhs, 16, seed=3)
+    pos[5] = pos[4]
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(16, 0.01, 0.0, max_level=12)
+    with pytest.raises(hhs.HHSError) as ei:
+        g.hold_step(d, 1.0 / 64)
+    assert ei.value.code == hhs.HHS_ERR_DEPTH

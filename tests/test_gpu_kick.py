@@ -1,0 +1,148 @@
+"""Kick parity: HIP kick (through the C ABI) vs the FP64 CPU oracle.
+
+Force-error metric (tolerance 1e-6, from the 1e-6 relative bar for forces):
+    err_i = |a_gpu,i - a_64,i| / sum_j m_j / (r_ij^2 + eps^2)
+i.e. the error relative to the sum of the magnitudes of the pair terms.
+A plain |da|/|a| is also reported, but a particle whose net force nearly
+cancels can exceed 1e-6 there without any term being wrong.
+"""
+import numpy as np
+import pytest
+
+from conftest import force_scale, plummer
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _dev(hhs, pos, vel, m):
+    return hhs.DeviceParticles.from_host(pos, vel, m)
+
+
+def _check(a_gpu, a_ref, scale):
+    err = np.linalg.norm(a_gpu - a_ref, axis=1) / scale
+    return float(err.max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257, 1000, 2048])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_kick_self_matches_oracle(hhs, oracle, torch_cuda, n, eps2):
+    pos, vel, m = plummer(hhs, n, seed=100 + n, eps2=eps2)
+    d = _dev(hhs, pos, vel, m)
+    dt, fac = 0.0078125, 0.5
+    hhs.kick_self(d, dt=dt, fac=fac, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    vel_ref, acc_ref = oracle.kick(pos, vel, pos, m, dt, fac, eps2)
+    acc = d.acc.cpu().numpy()
+    if n == 1:
+        assert np.all(acc == 0.0)
+        return
+    err = _check(acc, acc_ref, force_scale(pos, m, eps2))
+    assert err < TOL, f"n={n} eps2={eps2}: max normalised force error {err:.3e}"
+    dv = np.abs(d.vel.cpu().numpy() - vel_ref).max()
+    assert dv <= TOL * dt * fac * force_scale(pos, m, eps2).max() + 1e-15
+
+
+@pytest.mark.parametrize("s,w", [(1, 1), (1, 8), (2, 2), (2, 4), (4, 1), (4, 4), (4, 8)])
+def test_kick_every_config(hhs, oracle, torch_cuda, s, w):
+    n, eps2 = 777, 1e-8
+    pos, vel, m = plummer(hhs, n, seed=7, eps2=eps2)
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_set_config(s, w)
+    try:
+        hhs.kick_self(d, dt=1.0, fac=1.0, eps2=eps2)
+        torch_cuda.cuda.synchronize()
+    finally:
+        hhs.kick_set_config(0, 0)
+    acc_ref = oracle.accel(pos, pos, m, eps2)
+    err = _check(d.acc.cpu().numpy(), acc_ref, force_scale(pos, m, eps2))
+    assert err < TOL, f"S={s} W={w}: {err:.3e}"
+
+
+@pytest.mark.parametrize("n_sink,n_src", [(1, 5), (300, 517), (517, 300), (64, 1), (129, 4096)])
+def test_kick_cross_matches_oracle(hhs, oracle, torch_cuda, n_sink, n_src):
+    eps2 = 1e-8
+    ps, vs, ms = plummer(hhs, n_src, seed=11, eps2=eps2)
+    pk, vk, mk = plummer(hhs, n_sink, seed=12, eps2=eps2)
+    so, si = _dev(hhs, ps, vs, ms), _dev(hhs, pk, vk, mk)
+    dt = 0.25
+    hhs.kick_cross(so, si, dt=dt, fac=1.0, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    vel_ref, acc_ref = oracle.kick(pk, vk, ps, ms, dt, 1.0, eps2)
+    scale = np.array([(ms / (((ps - p) ** 2).sum(1) + eps2)).sum() for p in pk])
+    err = _check(si.acc.cpu().numpy(), acc_ref, scale)
+    assert err < TOL, f"{n_sink}<-{n_src}: {err:.3e}"
+
+
+def test_kick_zero_sources_leaves_velocity(hhs, torch_cuda):
+    pos, vel, m = plummer(hhs, 100, seed=3)
+    so, si = _dev(hhs, pos, vel, m), _dev(hhs, pos, vel, m)
+    hhs.kick_cross(so, si, n_src=0, dt=1.0)
+    torch_cuda.cuda.synchronize()
+    assert np.array_equal(si.vel.cpu().numpy(), vel)
+    assert np.all(si.acc.cpu().numpy() == 0.0)
+
+
+def test_kick_two_body_closed_form(hhs, torch_cuda):
+    # |a| = m / r^2 towards the other body
+    pos = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, 0.0]])
+    vel = np.zeros((2, 3))
+    m = np.array([0.25, 0.75])
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_self(d, eps2=0.0)
+    torch_cuda.cuda.synchronize()
+    a = d.acc.cpu().numpy()
+    r = 0.5
+    u = (pos[1] - pos[0]) / r
+    assert np.allclose(a[0], m[1] / r**2 * u, rtol=1e-6, atol=0)
+    assert np.allclose(a[1], -m[0] / r**2 * u, rtol=1e-6, atol=0)
+
+
+def test_kick_momentum_conserved(hhs, torch_cuda):
+    n, eps2 = 4096, 1e-8
+    pos, vel, m = plummer(hhs, n, seed=21, mass_model=1, eps2=eps2)
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_self(d, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    a = d.acc.cpu().numpy()
+    f = (m[:, None] * a).sum(0)
+    scale = (m[:, None] * np.abs(a)).sum(0)
+    assert np.all(np.abs(f) <= 1e-6 * scale)
+
+
+def test_kick_n16384_sampled(hhs, oracle, torch_cuda):
+    """Plummer N=16384, eps=1e-4, against the FP64 oracle on 512 sinks."""
+    n, eps2 = 16384, 1e-8
+    pos, vel, m = plummer(hhs, n, seed=2, eps2=eps2)
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_self(d, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    idx = np.random.default_rng(0).choice(n, 512, replace=False)
+    ref = oracle.accel_subset(pos, idx, pos, m, eps2)
+    err = _check(d.acc.cpu().numpy()[idx], ref, force_scale(pos, m, eps2, idx))
+    assert err < TOL, f"{err:.3e}"
+
+
+def test_kick_deterministic(hhs, torch_cuda):
+    pos, vel, m = plummer(hhs, 3000, seed=9, eps2=1e-8)
+    outs = []
+    for _ in range(2):
+        d = _dev(hhs, pos, vel, m)
+        hhs.kick_self(d, eps2=1e-8)
+        torch_cuda.cuda.synchronize()
+        outs.append(d.acc.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_kick_f32_storage(hhs, oracle, torch_cuda):
+    import torch
+    n, eps2 = 500, 1e-6
+    pos, vel, m = plummer(hhs, n, seed=31, eps2=eps2)
+    pos32 = pos.astype(np.float32).astype(np.float64)
+    d = hhs.DeviceParticles.from_host(pos32, vel, m, dtype=torch.float32)
+    hhs.kick_self(d, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    ref = oracle.accel(pos32, pos32, m.astype(np.float32).astype(np.float64), eps2)
+    err = _check(d.acc.cpu().numpy().astype(np.float64), ref, force_scale(pos32, m, eps2))
+    assert err < 1e-5  # FP32 storage: FP32 separations

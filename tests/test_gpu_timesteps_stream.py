@@ -1,0 +1,92 @@
+"""Timesteps, drift, update and energy: parity with the FP64 oracle.
+
+Timesteps, drift and update use the oracle's operation order with IEEE
+sqrt/divide and no FMA contraction, so they are compared bit for bit.
+Energy is compared to 1e-12 relative (the final sum order differs).
+"""
+import numpy as np
+import pytest
+
+from conftest import plummer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 65, 257, 1024, 3000])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_timesteps_bit_exact(hhs, oracle, torch_cuda, n, eps2):
+    pos, vel, m = plummer(hhs, n, seed=40 + n, mass_model=1, eps2=eps2)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=0.01, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    ref = oracle.timesteps(pos, vel, m, 0.01, eps2)
+    got = d.timestep.cpu().numpy()
+    assert np.array_equal(got, ref), f"max rel diff {np.max(np.abs(got / ref - 1)):.3e}"
+
+
+def test_timesteps_single_particle_and_cold_start(hhs, oracle, torch_cuda):
+    d = hhs.DeviceParticles.from_host(np.zeros((1, 3)), np.zeros((1, 3)), np.ones(1))
+    hhs.timesteps(d, eta=0.01)
+    torch_cuda.cuda.synchronize()
+    assert d.timestep.cpu().numpy()[0] == 1e30  # src/timestep.cuh:4 start value
+    # cold start: all velocities zero -> fly-by candidates are NaN/inf and never win
+    pos, _, m = plummer(hhs, 100, seed=5)
+    vel = np.zeros((100, 3))
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=0.02)
+    torch_cuda.cuda.synchronize()
+    assert np.array_equal(d.timestep.cpu().numpy(), oracle.timesteps(pos, vel, m, 0.02))
+
+
+def test_timesteps_circular_pair_closed_form(hhs, torch_cuda):
+    # v . r = 0: tau_ff = eta/sqrt2 * sqrt(r^3/M), tau_fb = eta * r / |dv|
+    r, M, eta = 0.5, 1.0, 0.03
+    v = np.sqrt(M / r)
+    pos = np.array([[-r / 2, 0, 0], [r / 2, 0, 0]])
+    vel = np.array([[0, -v / 2, 0], [0, v / 2, 0]])
+    m = np.array([0.5, 0.5])
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=eta)
+    torch_cuda.cuda.synchronize()
+    tff = eta / np.sqrt(2) * np.sqrt(r**3 / M)
+    tfb = eta * r / v
+    got = d.timestep.cpu().numpy()
+    assert np.allclose(got, min(tff, tfb), rtol=1e-14)
+
+
+def test_drift_bit_exact(hhs, oracle, torch_cuda):
+    torch = torch_cuda
+    n = 100003
+    rng = np.random.default_rng(0)
+    pos, vel = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    p = torch.as_tensor(pos).cuda()
+    v = torch.as_tensor(vel).cuda()
+    hhs.drift(p, v, dt=0.0123, fac=0.5)
+    torch.cuda.synchronize()
+    assert np.array_equal(p.cpu().numpy(), oracle.drift(pos, vel, 0.0123, 0.5))
+
+
+def test_update_inverse_permutation(hhs, oracle, torch_cuda):
+    torch = torch_cuda
+    n = 70001
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(n).astype(np.int32)
+    cur = hhs.DeviceParticles.from_host(rng.standard_normal((n, 3)), rng.standard_normal((n, 3)),
+                                        rng.random(n), rng.random(n))
+    cur.parent_id.copy_(torch.as_tensor(perm))
+    prev = hhs.DeviceParticles.empty(n)
+    hhs.update(prev, cur)
+    torch.cuda.synchronize()
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    for f in ("pos", "vel", "m", "timestep"):
+        assert np.array_equal(getattr(prev, f).cpu().numpy(), getattr(cur, f).cpu().numpy()[inv])
+
+
+@pytest.mark.parametrize("n,eps2", [(2, 0.0), (1000, 0.0), (5000, 1e-8)])
+def test_energy_matches_oracle(hhs, oracle, torch_cuda, n, eps2):
+    pos, vel, m = plummer(hhs, n, seed=n, eps2=eps2)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    ke, pe = hhs.energy(d.pos, d.vel, d.m, eps2)
+    assert abs(ke - oracle.kinetic(vel, m)) <= 1e-12 * abs(ke)
+    assert abs(pe - oracle.potential(pos, m, eps2)) <= 1e-12 * abs(pe)
