@@ -89,6 +89,11 @@ int hhs_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int hhs_memset(void *dst, int value, size_t bytes);
 int hhs_device_synchronize(void);
 int hhs_stream_synchronize(void *stream);
+/* Pinned host memory that kernels can write directly; stands in for the
+ * reference's cudaMallocManaged partition/params (src/particle_system.cuh:81,
+ * :123) so the host can read counts after a synchronisation. */
+int hhs_host_alloc_mapped(void **ptr, size_t bytes);
+int hhs_host_free_mapped(void *ptr);
 
 /* ------------------------------------------------------------- kicks -- */
 /* Direct-sum kick with Plummer softening eps2 = eps^2 and the attractive
@@ -138,6 +143,11 @@ int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2
  * that share one workspace must be ordered (same stream). */
 size_t hhs_scan_workspace_bytes(int n_max);
 int hhs_scan_workspace_init(void *ws, size_t bytes, void *stream);
+/* Library-owned, grow-only workspace of the calling thread, initialised and
+ * ready for n <= n_max (for callers that, like the reference's scan
+ * functions, carry no workspace).  Valid until the thread exits; calls from
+ * one thread must be stream-ordered. */
+int hhs_scan_workspace_thread(int n_max, void **ws, size_t *bytes);
 
 /* scan_predicate_{small,large,large_ev} (src/split.cuh:437-565):
  * exclusive prefix sum of n ints, in one pass (decoupled look-back). */

@@ -97,4 +97,17 @@ int hhs_stream_synchronize(void *stream) {
     return HHS_OK;
 }
 
+int hhs_host_alloc_mapped(void **ptr, size_t bytes) {
+    HHS_REQUIRE(ptr, "hhs_host_alloc_mapped: null output");
+    *ptr = nullptr;
+    if (bytes == 0) return HHS_OK;
+    HHS_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return HHS_OK;
+}
+
+int hhs_host_free_mapped(void *ptr) {
+    if (ptr) HHS_HIP(hipHostFree(ptr));
+    return HHS_OK;
+}
+
 }  // extern "C"

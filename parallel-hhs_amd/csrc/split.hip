@@ -387,6 +387,32 @@ int hhs_scan_workspace_init(void *ws, size_t bytes, void *stream) {
     return scan_ws_init(ws, bytes, as_stream(stream));
 }
 
+int hhs_scan_workspace_thread(int n_max, void **ws, size_t *bytes) {
+    struct ThreadWs {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+        ~ThreadWs() {
+            if (ptr) (void)hipFree(ptr);
+        }
+    };
+    static thread_local ThreadWs tw;
+    HHS_REQUIRE(ws && bytes, "scan_workspace_thread: null output");
+    const size_t need = scan_ws_bytes(n_max);
+    if (need > tw.bytes) {
+        // Wait for work queued on the old buffer before replacing it.
+        HHS_HIP(hipDeviceSynchronize());
+        if (tw.ptr) HHS_HIP(hipFree(tw.ptr));
+        tw.ptr = nullptr;
+        tw.bytes = 0;
+        HHS_HIP(hipMalloc(&tw.ptr, need));
+        HHS_HIP(hipMemset(tw.ptr, 0, need));
+        tw.bytes = need;
+    }
+    *ws = tw.ptr;
+    *bytes = tw.bytes;
+    return HHS_OK;
+}
+
 int hhs_scan_exclusive_i32(const int *in, int *out, int n, void *ws, size_t ws_bytes,
                            void *stream) {
     HHS_REQUIRE(n >= 0, "scan: negative n");

@@ -87,6 +87,8 @@ _SIGS = {
     "hhs_memset": (_i, [_vp, _i, _sz]),
     "hhs_device_synchronize": (_i, []),
     "hhs_stream_synchronize": (_i, [_vp]),
+    "hhs_host_alloc_mapped": (_i, [C.POINTER(_vp), _sz]),
+    "hhs_host_free_mapped": (_i, [_vp]),
     "hhs_kick_self_f64": (_i, [_P64, _i, _d, _d, _d, _vp]),
     "hhs_kick_self_f32": (_i, [_P32, _i, _f, _f, _f, _vp]),
     "hhs_kick_cross_f64": (_i, [_P64, _P64, _i, _i, _d, _d, _d, _vp]),
@@ -97,6 +99,7 @@ _SIGS = {
     "hhs_timesteps_f32": (_i, [_P32, _i, _f, _f, _vp]),
     "hhs_scan_workspace_bytes": (_sz, [_i]),
     "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
+    "hhs_scan_workspace_thread": (_i, [_i, C.POINTER(_vp), C.POINTER(_sz)]),
     "hhs_scan_exclusive_i32": (_i, [_vp, _vp, _i, _vp, _sz, _vp]),
     "hhs_split_f64": (_i, [_P64, _P64, _P64, _vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
     "hhs_split_f32": (_i, [_P32, _P32, _P32, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),

@@ -1,0 +1,118 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/hhs.h, rejects bad arguments with a status code and a
+message, and its host-only entry points (initial conditions) behave."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hhs.h")
+LIB = os.path.join(ROOT, "parallel-hhs_amd", "lib", "libhhs.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(hhs_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("hhs_kick_self_f64", "hhs_kick_cross_f64", "hhs_timesteps_f64",
+                 "hhs_split_f64", "hhs_scan_exclusive_i32", "hhs_drift_f64",
+                 "hhs_update_f64", "hhs_hold_step_f64", "hhs_hold_f64", "hhs_energy_f64",
+                 "hhs_globals_create", "hhs_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (hhs_[a-z0-9_]+)", out))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_binding_covers_every_declared_symbol(hhs):
+    missing = [n for n in declared_functions() if n not in hhs.EXPORTED]
+    assert not missing, f"hhs.py lacks: {missing}"
+    lib = hhs.raw()
+    for n in declared_functions():
+        assert getattr(lib, n) is not None
+
+
+def test_version_and_empty_error(hhs):
+    assert b"gfx950" in hhs.raw().hhs_version()
+
+
+def test_invalid_arguments_return_status(hhs):
+    import ctypes as C
+    lib = hhs.raw()
+    assert lib.hhs_kick_set_config(3, 1) == hhs.HHS_ERR_INVALID
+    assert "sinks_per_lane" in hhs.last_error()
+    assert lib.hhs_kick_set_config(2, 16) == hhs.HHS_ERR_INVALID
+    h = C.c_void_p()
+    assert lib.hhs_globals_create(C.byref(h), 0, 0.01, 0.0, 0) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_globals_create(C.byref(h), 10, -1.0, 0.0, 0) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_globals_create(C.byref(h), 10, 0.01, 0.0, 64) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_kick_self_f64(None, 10, 1.0, 1.0, 0.0, None) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_split_f64(None, None, None, None, None, 1.0, 5, None, None, 0,
+                             None) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_hold_step_f64(None, None, 1, 0, 0.0, 1.0, 1.0, 1, None) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_ic_plummer(0, 1, 0, 0, 0.0, None, None, None) == hhs.HHS_ERR_INVALID
+    assert "ic_plummer" in hhs.last_error()
+    assert lib.hhs_scan_workspace_bytes(0) >= 64
+    assert lib.hhs_scan_workspace_bytes(1 << 20) == 64 + 8 * (1 << 12)
+
+
+def test_kick_config_heuristic(hhs):
+    s, w = hhs.kick_get_config(131072, 131072)
+    assert (s, w) == (4, 4)
+    s, w = hhs.kick_get_config(100, 100)
+    assert s == 1 and 1 <= w <= 8
+    hhs.kick_set_config(2, 8)
+    assert hhs.kick_get_config(131072, 131072) == (2, 8)
+    hhs.kick_set_config(0, 0)
+    assert hhs.kick_get_config(131072, 131072) == (4, 4)
+
+
+# ------------------------------------------------------ initial conditions
+def test_plummer_deterministic_and_centred(hhs):
+    a = hhs.ic_plummer(2000, 42, 0, True, 0.0)
+    b = hhs.ic_plummer(2000, 42, 0, True, 0.0)
+    c = hhs.ic_plummer(2000, 43, 0, True, 0.0)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(a[0], c[0])
+    pos, vel, m = a
+    assert np.allclose(m, 1 / 2000)
+    assert np.abs((m[:, None] * pos).sum(0)).max() < 1e-14
+    assert np.abs((m[:, None] * vel).sum(0)).max() < 1e-14
+
+
+def test_plummer_henon_units_and_virial(hhs, oracle):
+    pos, vel, m = hhs.ic_plummer(4096, 7, 0, True, 0.0)
+    ke, pe = oracle.kinetic(vel, m), oracle.potential(pos, m)
+    assert ke / -pe == pytest.approx(0.5, rel=1e-12)
+    # Henon units: E = -1/4 for the model; a 4096-body sample is within a few %
+    assert ke + pe == pytest.approx(-0.25, rel=0.06)
+    r = np.sort(np.linalg.norm(pos, axis=1))
+    # Plummer half-mass radius 1.3048 a, a = 3 pi / 16 in Henon units
+    assert r[2047] == pytest.approx(1.3048 * 3 * np.pi / 16, rel=0.05)
+    # radii beyond 10 scale lengths are redrawn (the recentring shift is tiny)
+    assert r[-1] <= 10 * 3 * np.pi / 16 * 1.001
+
+
+def test_salpeter_masses(hhs):
+    _, _, m = hhs.ic_plummer(20000, 3, 1, False, 0.0)
+    assert m.sum() == pytest.approx(1.0, rel=1e-12)
+    ratio = m.max() / m.min()
+    assert 50 < ratio <= 1000.0 + 1e-9  # [0.1, 100] before normalisation
+    # slope of dN/dlog m: -1.35 for a Salpeter IMF
+    hist, edges = np.histogram(np.log10(m / m.min() * 0.1), bins=np.linspace(-1, 1, 9))
+    slope = np.polyfit(0.5 * (edges[1:] + edges[:-1]), np.log10(hist), 1)[0]
+    assert slope == pytest.approx(-1.35, abs=0.1)
