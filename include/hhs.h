@@ -136,6 +136,15 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
 int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,
                       void *stream);
 
+/* Timestep evaluation modes for the HOLD drivers. */
+#define HHS_TIMESTEP_EXACT 0 /* IEEE sqrt/divide, bit-exact with the formula   */
+#define HHS_TIMESTEP_FAST 1  /* rsqrt + Newton, one divide per particle; agrees
+                                with EXACT to a few ulp                         */
+/* Same criterion as hhs_timesteps_f64 in the FAST form (no counterpart in
+ * the reference). */
+int hhs_timesteps_fast_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
+                           void *stream);
+
 /* ------------------------------------------------------ scan / split -- */
 /* Device workspace for the single-pass scan (tile counter, epoch, tile
  * status words).  Allocate hhs_scan_workspace_bytes(n_max) bytes, call
@@ -226,6 +235,9 @@ typedef struct hhs_hold_stats {
     long long calls_per_level[64];
 } hhs_hold_stats;
 int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out);
+/* HHS_TIMESTEP_EXACT (default) or HHS_TIMESTEP_FAST for hold_step's
+ * timestep evaluations. */
+int hhs_globals_set_timestep_mode(hhs_globals *g, int mode);
 int hhs_globals_reset_stats(hhs_globals *g);
 
 /* hold_step<T,BLOCKSIZE> (src/hold.cuh:51-181) on total[0..n), at
@@ -241,6 +253,46 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
  * dt_root (timesteps recomputed at each root step) until t_end. */
 int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
                  void *stream);
+
+/* --------------------------------------------------------- ensembles -- */
+/* Many independent systems integrated in one persistent launch (no
+ * reference counterpart: the reference integrates one host-driven system).
+ * Each system runs the same HOLD recursion as hhs_hold_f64 (root steps of
+ * its dt_root until its t_end, timesteps recomputed at every root step),
+ * entirely on the device: a workgroup (or, for systems of at most
+ * wave_team_max_n particles, one wave) owns a system at a time.  A system's
+ * result does not depend on what else is in the batch. */
+typedef struct hhs_system_result {
+    double e0, e1;       /* total energy before / after (FP64, softened)     */
+    double t;            /* time reached                                     */
+    long long calls;     /* hold_step invocations                            */
+    long long interactions;
+    long long ts_pairs;
+    int max_level;
+    int status;          /* HHS_OK, HHS_ERR_DEPTH (level or call cap), ...   */
+} hhs_system_result;
+
+typedef struct hhs_ensemble_desc {
+    int n_systems;
+    const int *offsets;    /* device, n_systems + 1 particle offsets         */
+    double *pos, *vel;     /* device, AoS vec3, all systems back to back;
+                              updated in place, original order kept          */
+    const double *m;       /* device                                          */
+    const double *t_end;   /* device, per system                              */
+    const double *dt_root; /* device, per system                              */
+    double eta, eps;
+    int max_level;         /* <= 47; 0 selects 47                             */
+    long long max_calls;   /* per-system cap on hold_step calls, 0 = none     */
+    int wave_team_max_n;   /* 0 selects 256                                   */
+    hhs_system_result *results; /* device, per system                        */
+    int timestep_mode;     /* HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST         */
+} hhs_ensemble_desc;
+
+size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles);
+/* Asynchronous on `stream`; offsets_host is the host copy of d->offsets
+ * (used to order the work queues). */
+int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, void *ws,
+                         size_t ws_bytes, void *stream);
 
 /* ------------------------------------------- initial conditions (host) -- */
 /* Host-only (no GPU use).  Plummer sphere in Henon units (G = M = 1,

@@ -1,0 +1,601 @@
+// Batched HOLD: many independent systems integrated in one persistent launch.
+//
+// No reference counterpart (the reference integrates one system, host
+// driven, with a blocking sync per recursion call, src/split.cuh:430).  Here
+// a "team" -- a whole 256-thread workgroup, or one 64-lane wave for small
+// systems -- owns one system at a time and runs its entire HOLD integration
+// (root steps, the recursion of src/hold.cuh:51-181 as an explicit stack in
+// LDS, timesteps, splits, drifts, kicks) without returning to the host.
+//
+// * Work distribution: workgroups pull systems from two atomic queues --
+//   "big" systems (whole workgroup per system) first, then "small" systems
+//   (each wave pulls its own).  The host orders each queue largest-first.
+// * A system's numbers depend only on its own data and on its team size,
+//   which depends only on its particle count, so a system integrates to the
+//   same bits alone or in any mix.
+// * In-place hierarchy: a recursion level owns a contiguous index range
+//   [b, e) of the system's working copy; the split partitions it stably,
+//   fast first, through a scratch copy.  No scatter-back is needed; the
+//   original particle order is restored by id at the end.
+// * Same numerics as the per-system path: FP64 positions/velocities/drift,
+//   FP64-derived FP32 separations with two-sum compensated FP32 force sums,
+//   bit-exact FP64 pair timesteps (pair_math.h).
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "hhs_internal.h"
+#include "pair_math.h"
+
+namespace hhs {
+namespace {
+
+constexpr int kWG = 256;
+constexpr int kMaxFrames = 50;  // max_level <= 47 plus root and one child
+constexpr int kFold = 64;
+
+struct Frame {
+    int b, e, nf, stage;
+    double dt;
+    int calc, pad;
+};
+
+struct TeamShared {
+    Frame stack[kMaxFrames];
+    int sp;
+    int status;
+    int wave_tot[4];
+    double red[kWG];
+};
+
+struct EnsArgs {
+    const int *offsets;
+    double *pos, *vel;
+    const double *m;
+    const double *t_end, *dt_root;
+    double eta, c_ff, eps2;
+    float eps2f;
+    int max_level;
+    long long max_calls;
+    double *wpos, *wvel, *wm, *wts;
+    int *wid;
+    double *tpos, *tvel, *tm, *tts;
+    int *tid;
+    const int *big_list;
+    int n_big;
+    const int *small_list;
+    int n_small;
+    unsigned int *counters;
+    hhs_system_result *results;
+    int fast_ts;
+};
+
+template <int TEAM> struct Team;
+template <> struct Team<kWG> {
+    int t;
+    __device__ Team() : t(threadIdx.x) {}
+    __device__ void sync() const { __syncthreads(); }
+};
+template <> struct Team<64> {
+    int t;
+    __device__ Team() : t(threadIdx.x & 63) {}
+    __device__ void sync() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+// Sum of one int per thread over the team; every thread gets the total.
+template <int TEAM>
+__device__ int team_sum(int x, TeamShared &sh, const Team<TEAM> &tm) {
+    x = wave_sum(x);
+    if constexpr (TEAM == 64) {
+        return x;
+    } else {
+        if ((tm.t & 63) == 0) sh.wave_tot[tm.t >> 6] = x;
+        tm.sync();
+        const int s = sh.wave_tot[0] + sh.wave_tot[1] + sh.wave_tot[2] + sh.wave_tot[3];
+        tm.sync();
+        return s;
+    }
+}
+
+// Exclusive rank of p among the team's threads in thread order, and the
+// team total.
+template <int TEAM>
+__device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared &sh,
+                               const Team<TEAM> &tm) {
+    const int lane = tm.t & 63;
+    const uint64_t mask = __ballot(p);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int in_wave = __popcll(mask & lt);
+    if constexpr (TEAM == 64) {
+        excl = in_wave;
+        tot = __popcll(mask);
+    } else {
+        const int w = tm.t >> 6;
+        if (lane == 0) sh.wave_tot[w] = __popcll(mask);
+        tm.sync();
+        int before = 0, all = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = sh.wave_tot[k];
+            before += (k < w) ? c : 0;
+            all += c;
+        }
+        tm.sync();
+        excl = before + in_wave;
+        tot = all;
+    }
+}
+
+__device__ __forceinline__ void copy_rec(const EnsArgs &a, bool to_tmp, int src, int dst) {
+    const double *sp = to_tmp ? a.wpos : a.tpos;
+    const double *sv = to_tmp ? a.wvel : a.tvel;
+    const double *sm = to_tmp ? a.wm : a.tm;
+    const double *st = to_tmp ? a.wts : a.tts;
+    const int *si = to_tmp ? a.wid : a.tid;
+    double *dp = to_tmp ? a.tpos : a.wpos;
+    double *dv = to_tmp ? a.tvel : a.wvel;
+    double *dm = to_tmp ? a.tm : a.wm;
+    double *dt = to_tmp ? a.tts : a.wts;
+    int *di = to_tmp ? a.tid : a.wid;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dp[3 * dst + k] = sp[3 * src + k];
+        dv[3 * dst + k] = sv[3 * src + k];
+    }
+    dm[dst] = sm[src];
+    dt[dst] = st[src];
+    di[dst] = si[src];
+}
+
+// timesteps of particles [o+b, o+e) against each other (src/timestep.cuh).
+template <int TEAM>
+__device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, const Team<TEAM> &tm) {
+    const int jb = uni(o + b), je = uni(o + e);
+    for (int i0 = jb; i0 < je; i0 += TEAM) {
+        const int i = i0 + tm.t;
+        const int ic = i < je ? i : je - 1;
+        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
+        const double vxi = a.wvel[3 * ic], vyi = a.wvel[3 * ic + 1], vzi = a.wvel[3 * ic + 2];
+        const double mi = a.wm[ic];
+        double minval = TsConst<double>::huge;
+        if (a.fast_ts) {
+            double bden = 1.0;
+            for (int j = jb; j < je; ++j)
+                pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
+                                   a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
+                                   a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta, a.c_ff,
+                                   a.eps2, j != i, minval, bden);
+            minval = minval / bden;
+        } else {
+            for (int j = jb; j < je; ++j)
+                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
+                                      a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
+                                      a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta,
+                                      a.c_ff, a.eps2, j != i, minval);
+        }
+        if (i < je) a.wts[i] = minval;
+    }
+}
+
+// Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
+// the fast count after the lone-particle rule (src/split.cuh:321-324).
+template <int TEAM>
+__device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivot, TeamShared &sh,
+                              const Team<TEAM> &tm) {
+    int c = 0;
+    for (int i = o + b + tm.t; i < o + e; i += TEAM) c += (a.wts[i] <= pivot) ? 1 : 0;
+    const int nf = team_sum<TEAM>(c, sh, tm);
+    if (nf == 1) return 0;               // lone fast particle stays slow, order kept
+    if (nf == 0 || nf == e - b) return nf;  // stable partition is the identity
+    int fbase = 0, sbase = 0;
+    for (int c0 = b; c0 < e; c0 += TEAM) {
+        const int i = c0 + tm.t;
+        const bool p = i < e && a.wts[o + i] <= pivot;
+        int excl, tot;
+        team_scan_bits<TEAM>(p, excl, tot, sh, tm);
+        if (i < e) {
+            const int d = p ? (b + fbase + excl) : (b + nf + sbase + (i - c0) - excl);
+            copy_rec(a, true, o + i, o + d);
+        }
+        fbase += tot;
+        sbase += min(TEAM, e - c0) - tot;
+    }
+    tm.sync();
+    for (int i = o + b + tm.t; i < o + e; i += TEAM) copy_rec(a, false, i, i);
+    tm.sync();
+    return nf;
+}
+
+template <int TEAM>
+__device__ void team_drift(const EnsArgs &a, int o, int b, int e, double h,
+                           const Team<TEAM> &tm) {
+#pragma clang fp contract(off)
+    for (int k = 3 * (o + b) + tm.t; k < 3 * (o + e); k += TEAM)
+        a.wpos[k] = a.wpos[k] + a.wvel[k] * h;  // src/drift.cuh:25
+}
+
+// vel[i] += h * sum_j m_j (x_j - x_i) / (r^2 + eps2)^{3/2}, sinks [sb, se),
+// sources [jb, je) (absolute indices).
+template <int TEAM, bool SOFT>
+__device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, double h,
+                          const Team<TEAM> &tm) {
+    jb = uni(jb);
+    je = uni(je);
+    for (int i0 = uni(sb); i0 < se; i0 += TEAM) {
+        const int i = i0 + tm.t;
+        const int ic = i < se ? i : se - 1;
+        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
+        float hx = 0.f, lx = 0.f, hy = 0.f, ly = 0.f, hz = 0.f, lz = 0.f;
+        for (int j0 = jb; j0 < je; j0 += kFold) {
+            const int j1 = min(j0 + kFold, je);
+            float px = 0.f, py = 0.f, pz = 0.f;
+#pragma unroll 4
+            for (int j = j0; j < j1; ++j)
+                pair_accel<SOFT>(a.wpos[3 * j], a.wpos[3 * j + 1], a.wpos[3 * j + 2],
+                                 static_cast<float>(a.wm[j]), xi, yi, zi, a.eps2f, px, py, pz);
+            two_sum_add(hx, lx, px);
+            two_sum_add(hy, ly, py);
+            two_sum_add(hz, lz, pz);
+        }
+        if (i < se) {
+            const double ax = static_cast<double>(hx) + static_cast<double>(lx);
+            const double ay = static_cast<double>(hy) + static_cast<double>(ly);
+            const double az = static_cast<double>(hz) + static_cast<double>(lz);
+            a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
+            a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
+            a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
+        }
+    }
+}
+
+// E = sum 1/2 m v^2 - sum_{i<j} m_i m_j / sqrt(r^2 + eps2), FP64, fixed
+// reduction order.
+template <int TEAM>
+__device__ double team_energy(const EnsArgs &a, int o, int n, TeamShared &sh,
+                              const Team<TEAM> &tm) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    const int jb = uni(o), je = uni(o + n);
+    for (int i0 = jb; i0 < je; i0 += TEAM) {
+        const int i = i0 + tm.t;
+        const int ic = i < je ? i : je - 1;
+        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
+        double s = 0.0;
+        for (int j = i0 + 1; j < je; ++j) {
+            const double dx = a.wpos[3 * j] - xi, dy = a.wpos[3 * j + 1] - yi,
+                         dz = a.wpos[3 * j + 2] - zi;
+            const double r2 = dx * dx + dy * dy + dz * dz + a.eps2;
+            const double t = a.wm[j] / sqrt(r2);
+            if (j > i && r2 != 0.0) s += t;
+        }
+        if (i < je) {
+            const double vx = a.wvel[3 * i], vy = a.wvel[3 * i + 1], vz = a.wvel[3 * i + 2];
+            acc += 0.5 * a.wm[i] * (vx * vx + vy * vy + vz * vz) - a.wm[i] * s;
+        }
+    }
+    sh.red[tm.t] = acc;
+    tm.sync();
+    double tot = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < TEAM; ++k) tot += sh.red[k];
+    tm.sync();
+    return tot;
+}
+
+struct SysStats {
+    long long calls = 0, interactions = 0, ts_pairs = 0;
+    int max_level = 0;
+};
+
+// One HOLD root step of dt over the whole system (src/hold.cuh:51-181 with
+// an explicit stack).  Returns HHS_OK or a status code (uniform).
+template <int TEAM, bool SOFT>
+__device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamShared &sh,
+                              const Team<TEAM> &tm, SysStats &st) {
+    if (tm.t == 0) {
+        sh.sp = 1;
+        sh.status = HHS_OK;
+        sh.stack[0] = Frame{0, n, 0, 0, dt, 1, 0};
+    }
+    tm.sync();
+    for (;;) {
+        const int sp = sh.sp;
+        if (sp == 0 || sh.status != HHS_OK) break;
+        const Frame f = sh.stack[sp - 1];
+        const int level = sp - 1;
+        tm.sync();
+        if (f.stage == 0) {
+            if (level > a.max_level || (a.max_calls > 0 && st.calls >= a.max_calls)) {
+                if (tm.t == 0) sh.status = HHS_ERR_DEPTH;
+                tm.sync();
+                break;
+            }
+            st.calls++;
+            st.max_level = max(st.max_level, level);
+            if (f.calc) {  // src/hold.cuh:74-76
+                team_timesteps<TEAM>(a, o, f.b, f.e, tm);
+                st.ts_pairs += static_cast<long long>(f.e - f.b) * (f.e - f.b - 1);
+                tm.sync();
+            }
+            const int nf = team_partition<TEAM>(a, o, f.b, f.e, f.dt, sh, tm);  // :79-83
+            if (tm.t == 0) {
+                sh.stack[sp - 1].nf = nf;
+                sh.stack[sp - 1].stage = 1;
+                if (nf > 0) {  // :116-121
+                    sh.stack[sp] = Frame{f.b, f.b + nf, 0, 0, f.dt / 2, 0, 0};
+                    sh.sp = sp + 1;
+                }
+            }
+            tm.sync();
+        } else if (f.stage == 1) {
+            const int sb = f.b + f.nf;
+            if (f.e > sb) {
+                team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);  // :123-124
+                tm.sync();
+                // :137-145 -- slow sinks from every particle, fast sinks from slow ones
+                team_kick<TEAM, SOFT>(a, o + sb, o + f.e, o + f.b, o + f.e, f.dt, tm);
+                if (f.nf > 0) team_kick<TEAM, SOFT>(a, o + f.b, o + sb, o + sb, o + f.e, f.dt, tm);
+                st.interactions += static_cast<long long>(f.e - sb) * (f.e - f.b) +
+                                   static_cast<long long>(f.nf) * (f.e - sb);
+                tm.sync();
+                team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);  // :158-159
+                tm.sync();
+            }
+            if (tm.t == 0) {
+                sh.stack[sp - 1].stage = 2;
+                if (f.nf > 0) {  // :168-173
+                    sh.stack[sp] = Frame{f.b, sb, 0, 0, f.dt / 2, 1, 0};
+                    sh.sp = sp + 1;
+                }
+            }
+            tm.sync();
+        } else {
+            if (tm.t == 0) sh.sp = sp - 1;
+            tm.sync();
+        }
+    }
+    const int status = sh.status;
+    tm.sync();
+    return status;
+}
+
+template <int TEAM, bool SOFT>
+__device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, const Team<TEAM> &tm) {
+    const int o = uni(a.offsets[s]);
+    const int n = uni(a.offsets[s + 1]) - o;
+    for (int i = tm.t; i < n; i += TEAM) {
+        const int g = o + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a.wpos[3 * g + k] = a.pos[3 * g + k];
+            a.wvel[3 * g + k] = a.vel[3 * g + k];
+        }
+        a.wm[g] = a.m[g];
+        a.wts[g] = 0.0;
+        a.wid[g] = i;
+    }
+    tm.sync();
+    SysStats st;
+    int status = HHS_OK;
+    double e0 = 0.0, e1 = 0.0, t = 0.0;
+    if (n > 0) {
+        e0 = team_energy<TEAM>(a, o, n, sh, tm);
+        const double t_end = a.t_end[s], dtr = a.dt_root[s];
+        if (!(dtr > 0.0)) status = HHS_ERR_INVALID;
+        while (status == HHS_OK && t < t_end) {
+            status = team_hold_root<TEAM, SOFT>(a, o, n, dtr, sh, tm, st);
+            if (status == HHS_OK) t += dtr;
+        }
+        e1 = team_energy<TEAM>(a, o, n, sh, tm);
+        // restore the original particle order
+        for (int i = tm.t; i < n; i += TEAM) {
+            const int g = o + i;
+            const int u = o + a.wid[g];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a.pos[3 * u + k] = a.wpos[3 * g + k];
+                a.vel[3 * u + k] = a.wvel[3 * g + k];
+            }
+        }
+    }
+    if (tm.t == 0) {
+        hhs_system_result r;
+        r.e0 = e0;
+        r.e1 = e1;
+        r.t = t;
+        r.calls = st.calls;
+        r.interactions = st.interactions;
+        r.ts_pairs = st.ts_pairs;
+        r.max_level = st.max_level;
+        r.status = status;
+        a.results[s] = r;
+    }
+    tm.sync();
+}
+
+template <bool SOFT>
+__global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
+    __shared__ TeamShared sh[4];
+    __shared__ int s_item;
+    // Big systems: the whole workgroup on one system.
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_item = static_cast<int>(__hip_atomic_fetch_add(&a.counters[0], 1u, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT));
+        __syncthreads();
+        const int k = s_item;
+        __syncthreads();
+        if (k >= a.n_big) break;
+        Team<kWG> tm;
+        team_run_system<kWG, SOFT>(a, a.big_list[k], sh[0], tm);
+    }
+    // Small systems: each wave pulls its own.
+    const int w = threadIdx.x >> 6;
+    Team<64> tw;
+    for (;;) {
+        int k = 0;
+        if (tw.t == 0)
+            k = static_cast<int>(__hip_atomic_fetch_add(&a.counters[1], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT));
+        k = __shfl(k, 0, 64);
+        if (k >= a.n_small) break;
+        team_run_system<64, SOFT>(a, a.small_list[k], sh[w], tw);
+    }
+}
+
+__global__ void zero_counters_kernel(unsigned int *c) {
+    if (threadIdx.x < 16) c[threadIdx.x] = 0u;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+struct EnsLayout {
+    size_t counters, big, small, wpos, wvel, wm, wts, wid, tpos, tvel, tm, tts, tid, total;
+};
+
+EnsLayout layout(int nsys, long long np) {
+    EnsLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t r = off;
+        off += align256(bytes);
+        return r;
+    };
+    const size_t n = static_cast<size_t>(np > 0 ? np : 1);
+    const size_t s = static_cast<size_t>(nsys > 0 ? nsys : 1);
+    L.counters = take(64);
+    L.big = take(s * 4);
+    L.small = take(s * 4);
+    L.wpos = take(3 * n * 8);
+    L.wvel = take(3 * n * 8);
+    L.wm = take(n * 8);
+    L.wts = take(n * 8);
+    L.wid = take(n * 4);
+    L.tpos = take(3 * n * 8);
+    L.tvel = take(3 * n * 8);
+    L.tm = take(n * 8);
+    L.tts = take(n * 8);
+    L.tid = take(n * 4);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+}  // namespace hhs
+
+using namespace hhs;
+
+extern "C" {
+
+size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles) {
+    return layout(n_systems, n_particles).total;
+}
+
+int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, void *ws,
+                         size_t ws_bytes, void *stream) {
+    HHS_REQUIRE(d && offsets_host, "ensemble: null descriptor or offsets");
+    const int nsys = d->n_systems;
+    HHS_REQUIRE(nsys >= 0, "ensemble: negative system count");
+    if (nsys == 0) return HHS_OK;
+    HHS_REQUIRE(d->offsets && d->pos && d->vel && d->m && d->t_end && d->dt_root && d->results,
+                "ensemble: null device array in descriptor");
+    HHS_REQUIRE(d->eta > 0.0 && d->eps >= 0.0, "ensemble: need eta > 0 and eps >= 0");
+    const int max_level = d->max_level > 0 ? d->max_level : 47;
+    HHS_REQUIRE(max_level <= kMaxFrames - 3, "ensemble: max_level must be <= %d",
+                kMaxFrames - 3);
+    HHS_REQUIRE(offsets_host[0] == 0, "ensemble: offsets must start at 0");
+    for (int s = 0; s < nsys; ++s)
+        HHS_REQUIRE(offsets_host[s + 1] >= offsets_host[s], "ensemble: offsets not monotone");
+    const long long np = offsets_host[nsys];
+    const EnsLayout L = layout(nsys, np);
+    HHS_REQUIRE(ws && ws_bytes >= L.total, "ensemble: workspace of %zu bytes, need %zu", ws_bytes,
+                L.total);
+    const int wave_max = d->wave_team_max_n > 0 ? d->wave_team_max_n : 256;
+
+    // Largest-first queues: whole-workgroup systems and one-wave systems.
+    std::vector<int> big, small;
+    for (int s = 0; s < nsys; ++s)
+        (offsets_host[s + 1] - offsets_host[s] > wave_max ? big : small).push_back(s);
+    auto by_size = [&](int x, int y) {
+        const int nx = offsets_host[x + 1] - offsets_host[x];
+        const int ny = offsets_host[y + 1] - offsets_host[y];
+        return nx != ny ? nx > ny : x < y;
+    };
+    std::sort(big.begin(), big.end(), by_size);
+    std::sort(small.begin(), small.end(), by_size);
+
+    char *w = static_cast<char *>(ws);
+    hipStream_t st = as_stream(stream);
+    if (!big.empty())
+        HHS_HIP(hipMemcpy(w + L.big, big.data(), big.size() * 4, hipMemcpyHostToDevice));
+    if (!small.empty())
+        HHS_HIP(hipMemcpy(w + L.small, small.data(), small.size() * 4, hipMemcpyHostToDevice));
+    unsigned int *counters = reinterpret_cast<unsigned int *>(w + L.counters);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, counters);
+    HHS_LAUNCH_CHECK("zero_counters_kernel");
+
+    EnsArgs a;
+    a.offsets = d->offsets;
+    a.pos = d->pos;
+    a.vel = d->vel;
+    a.m = d->m;
+    a.t_end = d->t_end;
+    a.dt_root = d->dt_root;
+    a.eta = d->eta;
+    a.c_ff = d->eta / TsConst<double>::sqrt2;
+    a.eps2 = d->eps * d->eps;
+    a.eps2f = static_cast<float>(a.eps2);
+    a.max_level = max_level;
+    a.max_calls = d->max_calls;
+    a.wpos = reinterpret_cast<double *>(w + L.wpos);
+    a.wvel = reinterpret_cast<double *>(w + L.wvel);
+    a.wm = reinterpret_cast<double *>(w + L.wm);
+    a.wts = reinterpret_cast<double *>(w + L.wts);
+    a.wid = reinterpret_cast<int *>(w + L.wid);
+    a.tpos = reinterpret_cast<double *>(w + L.tpos);
+    a.tvel = reinterpret_cast<double *>(w + L.tvel);
+    a.tm = reinterpret_cast<double *>(w + L.tm);
+    a.tts = reinterpret_cast<double *>(w + L.tts);
+    a.tid = reinterpret_cast<int *>(w + L.tid);
+    a.big_list = reinterpret_cast<const int *>(w + L.big);
+    a.n_big = static_cast<int>(big.size());
+    a.small_list = reinterpret_cast<const int *>(w + L.small);
+    a.n_small = static_cast<int>(small.size());
+    a.counters = counters;
+    a.results = d->results;
+    HHS_REQUIRE(d->timestep_mode == HHS_TIMESTEP_EXACT || d->timestep_mode == HHS_TIMESTEP_FAST,
+                "ensemble: bad timestep_mode %d", d->timestep_mode);
+    a.fast_ts = d->timestep_mode == HHS_TIMESTEP_FAST;
+
+    // Persistent grid: enough workgroups for every big system and every
+    // group of four small ones, capped at 8 per CU.
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            v > 0)
+            cus = v;
+    }
+    const long long items = static_cast<long long>(big.size()) + (small.size() + 3) / 4;
+    const int grid = static_cast<int>(std::max(1LL, std::min<long long>(items, 8LL * cus)));
+    if (a.eps2 > 0.0)
+        hipLaunchKernelGGL(ensemble_kernel<true>, dim3(grid), dim3(kWG), 0, st, a);
+    else
+        hipLaunchKernelGGL(ensemble_kernel<false>, dim3(grid), dim3(kWG), 0, st, a);
+    HHS_LAUNCH_CHECK("ensemble_kernel");
+    return HHS_OK;
+}
+
+}  // extern "C"

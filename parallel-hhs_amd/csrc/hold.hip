@@ -43,7 +43,7 @@ struct hhs_globals {
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
-    bool ws_ready = false;
+    int timestep_mode = HHS_TIMESTEP_EXACT;
 };
 
 namespace hhs {
@@ -106,8 +106,11 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     int rc;
     // src/hold.cuh:74-76
     if (calc) {
-        rc = launch_timesteps<double>(total.pos, total.vel, total.m, total.timestep, n, g->eta,
-                                      g->eps2, st);
+        rc = g->timestep_mode == HHS_TIMESTEP_FAST
+                 ? launch_timesteps_fast(total.pos, total.vel, total.m, total.timestep, n, g->eta,
+                                         g->eps2, st)
+                 : launch_timesteps<double>(total.pos, total.vel, total.m, total.timestep, n,
+                                            g->eta, g->eps2, st);
         if (rc) return rc;
         S.ts_pairs += static_cast<long long>(n) * (n - 1);
     }
@@ -219,6 +222,14 @@ void hhs_globals_set_simtime(hhs_globals *g, double t) {
 }
 double hhs_globals_eta(const hhs_globals *g) { return g ? g->eta : 0.0; }
 double hhs_globals_eps(const hhs_globals *g) { return g ? g->eps : 0.0; }
+
+int hhs_globals_set_timestep_mode(hhs_globals *g, int mode) {
+    HHS_REQUIRE(g, "set_timestep_mode: null globals");
+    HHS_REQUIRE(mode == HHS_TIMESTEP_EXACT || mode == HHS_TIMESTEP_FAST,
+                "set_timestep_mode: mode must be HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST");
+    g->timestep_mode = mode;
+    return HHS_OK;
+}
 
 int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out) {
     HHS_REQUIRE(g && out, "globals_stats: null argument");

@@ -22,25 +22,12 @@
 #include <atomic>
 
 #include "hhs_internal.h"
+#include "pair_math.h"
 
 namespace hhs {
 namespace {
 
 constexpr int kFold = 64;  // sources per plain-FP32 block before a two-sum fold
-
-// Knuth two-sum of hi + p; the rounding error is carried into lo.
-__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float p) {
-    const float s = hi + p;
-    const float bp = s - hi;
-    const float err = (hi - (s - bp)) + (p - bp);
-    hi = s;
-    lo += err;
-}
-
-__device__ __forceinline__ float separation(double xj, double xi) {
-    return static_cast<float>(xj - xi);
-}
-__device__ __forceinline__ float separation(float xj, float xi) { return xj - xi; }
 
 template <typename T, int S, bool SOFT>
 __device__ __forceinline__ void accumulate_range(const T *__restrict__ sp,
@@ -61,19 +48,8 @@ __device__ __forceinline__ void accumulate_range(const T *__restrict__ sp,
             const T zj = sp[3 * j + 2];
             const float mj = static_cast<float>(sm[j]);
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const float dx = separation(xj, xi[s]);
-                const float dy = separation(yj, yi[s]);
-                const float dz = separation(zj, zi[s]);
-                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-                const float rinv = __builtin_amdgcn_rsqf(r2);
-                const float rinv2 = rinv * rinv;
-                float w = (mj * rinv) * rinv2;
-                if (!SOFT) w = (r2 > 0.0f) ? w : 0.0f;  // src/kick.cuh:25 (skip d2 == 0)
-                px[s] = __builtin_fmaf(w, dx, px[s]);
-                py[s] = __builtin_fmaf(w, dy, py[s]);
-                pz[s] = __builtin_fmaf(w, dz, pz[s]);
-            }
+            for (int s = 0; s < S; ++s)
+                pair_accel<SOFT>(xj, yj, zj, mj, xi[s], yi[s], zi[s], eps2, px[s], py[s], pz[s]);
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) {

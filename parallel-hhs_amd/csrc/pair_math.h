@@ -1,0 +1,137 @@
+// Device-side pair formulas shared by the per-system kernels and the
+// batched (ensemble) kernel, so both paths produce identical numbers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hhs {
+
+template <typename T> struct TsConst;
+template <> struct TsConst<double> {
+    static constexpr double sqrt2 = 1.41421356237309504880;
+    static constexpr double huge = 1.e30;
+};
+template <> struct TsConst<float> {
+    static constexpr float sqrt2 = 1.41421356237309504880f;
+    static constexpr float huge = 1.e30f;
+};
+
+// One (i, j) pair of the timestep criterion, folded into minval.
+// Reference: _get_pp_timestep, src/timestep.cuh:17-60, in its operation
+// order (dr = x_i - x_j, dot as dv.x*dr.x + dv.y*dr.y + dv.z*dr.z,
+// src/vec.cuh:31,44), with r^2 -> r^2 + eps2.  `other` is j != i.  IEEE
+// sqrt/divide and no contraction: bit-exact with an FP64 CPU evaluation.
+template <typename T>
+__device__ __forceinline__ void pair_timestep(T xi, T yi, T zi, T vxi, T vyi, T vzi, T mi, T xj,
+                                              T yj, T zj, T vxj, T vyj, T vzj, T mj, T eta,
+                                              T c_ff, T eps2, bool other, T &minval) {
+#pragma clang fp contract(off)
+    const T drx = xi - xj, dry = yi - yj, drz = zi - zj;
+    const T dvx = vxi - vxj, dvy = vyi - vyj, dvz = vzi - vzj;
+    const T dr2 = drx * drx + dry * dry + drz * drz + eps2;
+    const T dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+    const T r = sqrt(dr2);
+    const T vdotdr2 = (dvx * drx + dvy * dry + dvz * drz) / dr2;
+    // free-fall (src/timestep.cuh:36-42); c_ff = eta / sqrt(2)
+    T tau = c_ff * r * sqrt(r / (mj + mi));
+    T dtau = T(3) * tau * vdotdr2 / T(2);
+    dtau = (dtau < T(1)) ? dtau : T(1);
+    tau /= (T(1) - dtau / T(2));
+    if (other && tau < minval) minval = tau;
+    // fly-by (src/timestep.cuh:46-57)
+    tau = eta * r / sqrt(dv2);
+    dtau = tau * vdotdr2 * (T(1) + (mi + mj) / (dv2 * r));
+    dtau = (dtau < T(1)) ? dtau : T(1);
+    tau /= (T(1) - dtau / T(2));
+    if (other && tau < minval) minval = tau;
+}
+
+// 1/sqrt(x) in FP64: the hardware estimate (v_rsq_f64) refined by two
+// Newton-Raphson steps y <- y (3 - x y^2) / 2, which takes the estimate to
+// full double precision.  x = 0 gives +inf and x = inf gives 0, as 1/sqrt.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    if (x > 0.0 && x < __builtin_inf()) {
+        y = y * __builtin_fma(-hx * y, y, 1.5);
+        y = y * __builtin_fma(-hx * y, y, 1.5);
+    }
+    return y;
+}
+
+// Fast form of pair_timestep (same criterion, same NaN/inf behaviour): the
+// square roots come from rsqrt_nr and the two divisions by (1 - dtau/2) are
+// never performed -- each candidate is kept as a fraction num/den with
+// den >= 1/2 and the running minimum is compared by cross-multiplication;
+// the caller divides once per sink at the end (bnum / bden).  Agrees with the
+// IEEE form to a few ulp.
+__device__ __forceinline__ void fold_fraction(double num, double den, bool other, double &bnum,
+                                              double &bden) {
+    if (other && num * bden < bnum * den) {
+        bnum = num;
+        bden = den;
+    }
+}
+
+__device__ __forceinline__ void pair_timestep_fast(double xi, double yi, double zi, double vxi,
+                                                   double vyi, double vzi, double mi, double xj,
+                                                   double yj, double zj, double vxj, double vyj,
+                                                   double vzj, double mj, double eta, double c_ff,
+                                                   double eps2, bool other, double &bnum,
+                                                   double &bden) {
+    const double drx = xi - xj, dry = yi - yj, drz = zi - zj;
+    const double dvx = vxi - vxj, dvy = vyi - vyj, dvz = vzi - vzj;
+    const double dr2 = __builtin_fma(drx, drx, __builtin_fma(dry, dry, __builtin_fma(drz, drz, eps2)));
+    const double dv2 = __builtin_fma(dvx, dvx, __builtin_fma(dvy, dvy, dvz * dvz));
+    const double rv = __builtin_fma(dvx, drx, __builtin_fma(dvy, dry, dvz * drz));
+    const double mu = mj + mi;
+    const double rinv = rsqrt_nr(dr2);
+    const double r = dr2 > 0.0 ? dr2 * rinv : 0.0;  // sqrt(dr2), exact 0 at 0
+    const double u = rv * (rinv * rinv);               // (dr.dv) / r^2
+    // free-fall: c_ff * r * sqrt(r / mu) = c_ff / sqrt(mu / r^3)
+    const double tff = c_ff * rsqrt_nr(mu * rinv * (rinv * rinv));
+    double d = 1.5 * tff * u;
+    d = (d < 1.0) ? d : 1.0;
+    fold_fraction(tff, __builtin_fma(-0.5, d, 1.0), other, bnum, bden);
+    // fly-by: eta * r / |dv|
+    const double vinv = rsqrt_nr(dv2);
+    const double tfb = eta * r * vinv;
+    d = tfb * u * __builtin_fma(mu * (vinv * vinv), rinv, 1.0);
+    d = (d < 1.0) ? d : 1.0;
+    fold_fraction(tfb, __builtin_fma(-0.5, d, 1.0), other, bnum, bden);
+}
+
+// Knuth two-sum of hi + p; the rounding error is carried into lo.
+__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float p) {
+    const float s = hi + p;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (p - bp);
+    hi = s;
+    lo += err;
+}
+
+__device__ __forceinline__ float separation(double xj, double xi) {
+    return static_cast<float>(xj - xi);
+}
+__device__ __forceinline__ float separation(float xj, float xi) { return xj - xi; }
+
+// One source's contribution to a sink's acceleration (FP32 from FP64-derived
+// separations); SOFT = (eps2 > 0), else pairs with r^2 == 0 are skipped
+// (src/kick.cuh:25).
+template <bool SOFT, typename T>
+__device__ __forceinline__ void pair_accel(T xj, T yj, T zj, float mj, T xi, T yi, T zi,
+                                           float eps2, float &px, float &py, float &pz) {
+    const float dx = separation(xj, xi);
+    const float dy = separation(yj, yi);
+    const float dz = separation(zj, zi);
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    float w = (mj * rinv) * rinv2;
+    if (!SOFT) w = (r2 > 0.0f) ? w : 0.0f;
+    px = __builtin_fmaf(w, dx, px);
+    py = __builtin_fmaf(w, dy, py);
+    pz = __builtin_fmaf(w, dz, pz);
+}
+
+}  // namespace hhs

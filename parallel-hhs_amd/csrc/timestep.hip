@@ -15,19 +15,10 @@
 // :89,:95) has no counterpart: there are no barriers here, and lanes past
 // n compute on a clamped sink and do not store.
 #include "hhs_internal.h"
+#include "pair_math.h"
 
 namespace hhs {
 namespace {
-
-template <typename T> struct TsConst;
-template <> struct TsConst<double> {
-    static constexpr double sqrt2 = 1.41421356237309504880;
-    static constexpr double huge = 1.e30;
-};
-template <> struct TsConst<float> {
-    static constexpr float sqrt2 = 1.41421356237309504880f;
-    static constexpr float huge = 1.e30f;
-};
 
 template <typename T>
 __global__ __launch_bounds__(256) void timesteps_kernel(const T *__restrict__ pos,
@@ -35,7 +26,6 @@ __global__ __launch_bounds__(256) void timesteps_kernel(const T *__restrict__ po
                                                         const T *__restrict__ m,
                                                         T *__restrict__ ts, int n, T eta,
                                                         T eps2) {
-#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int ic = i < n ? i : n - 1;
     const T xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
@@ -43,29 +33,33 @@ __global__ __launch_bounds__(256) void timesteps_kernel(const T *__restrict__ po
     const T mi = m[ic];
     const T c_ff = eta / TsConst<T>::sqrt2;
     T minval = TsConst<T>::huge;
-    for (int j = 0; j < n; ++j) {
-        const T drx = xi - pos[3 * j], dry = yi - pos[3 * j + 1], drz = zi - pos[3 * j + 2];
-        const T dvx = vxi - vel[3 * j], dvy = vyi - vel[3 * j + 1], dvz = vzi - vel[3 * j + 2];
-        const T mj = m[j];
-        const T dr2 = drx * drx + dry * dry + drz * drz + eps2;
-        const T dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
-        const T r = sqrt(dr2);
-        const T vdotdr2 = (dvx * drx + dvy * dry + dvz * drz) / dr2;
-        // free-fall (src/timestep.cuh:36-42)
-        T tau = c_ff * r * sqrt(r / (mj + mi));
-        T dtau = T(3) * tau * vdotdr2 / T(2);
-        dtau = (dtau < T(1)) ? dtau : T(1);
-        tau /= (T(1) - dtau / T(2));
-        const bool other = (j != i);
-        if (other && tau < minval) minval = tau;
-        // fly-by (src/timestep.cuh:46-57)
-        tau = eta * r / sqrt(dv2);
-        dtau = tau * vdotdr2 * (T(1) + (mi + mj) / (dv2 * r));
-        dtau = (dtau < T(1)) ? dtau : T(1);
-        tau /= (T(1) - dtau / T(2));
-        if (other && tau < minval) minval = tau;
-    }
+    for (int j = 0; j < n; ++j)
+        pair_timestep<T>(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
+                         vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta, c_ff, eps2, j != i,
+                         minval);
     if (i < n) ts[i] = minval;
+}
+
+// Fast FP64 form (pair_timestep_fast): rsqrt + Newton instead of IEEE
+// sqrt/divide, one division per sink.  Agrees with timesteps_kernel to a
+// few ulp; not bit-exact.
+__global__ __launch_bounds__(256) void timesteps_fast_kernel(const double *__restrict__ pos,
+                                                             const double *__restrict__ vel,
+                                                             const double *__restrict__ m,
+                                                             double *__restrict__ ts, int n,
+                                                             double eta, double eps2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ic = i < n ? i : n - 1;
+    const double xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
+    const double vxi = vel[3 * ic], vyi = vel[3 * ic + 1], vzi = vel[3 * ic + 2];
+    const double mi = m[ic];
+    const double c_ff = eta / TsConst<double>::sqrt2;
+    double bnum = TsConst<double>::huge, bden = 1.0;
+    for (int j = 0; j < n; ++j)
+        pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
+                           pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta,
+                           c_ff, eps2, j != i, bnum, bden);
+    if (i < n) ts[i] = bnum / bden;
 }
 
 }  // namespace
@@ -79,6 +73,17 @@ int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta
     dim3 block(256), grid((n + 255) / 256);
     hipLaunchKernelGGL((timesteps_kernel<T>), grid, block, 0, s, pos, vel, m, ts, n, eta, eps2);
     HHS_LAUNCH_CHECK("timesteps_kernel");
+    return HHS_OK;
+}
+
+int launch_timesteps_fast(const double *pos, const double *vel, const double *m, double *ts,
+                          int n, double eta, double eps2, hipStream_t s) {
+    HHS_REQUIRE(n >= 0, "timesteps: negative n");
+    if (n == 0) return HHS_OK;
+    HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
+    dim3 block(256), grid((n + 255) / 256);
+    hipLaunchKernelGGL(timesteps_fast_kernel, grid, block, 0, s, pos, vel, m, ts, n, eta, eps2);
+    HHS_LAUNCH_CHECK("timesteps_fast_kernel");
     return HHS_OK;
 }
 
@@ -99,6 +104,14 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
     HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
     return launch_timesteps<double>(sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
                                     as_stream(stream));
+}
+
+int hhs_timesteps_fast_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
+                           void *stream) {
+    HHS_REQUIRE(sys, "timesteps: null system");
+    HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
+    return launch_timesteps_fast(sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
+                                 as_stream(stream));
 }
 
 int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,

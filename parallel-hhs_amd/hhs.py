@@ -66,6 +66,21 @@ class Partition(C.Structure):
     _fields_ = [("N_slow", C.c_int), ("N_fast", C.c_int)]
 
 
+class SystemResult(C.Structure):
+    _fields_ = [("e0", C.c_double), ("e1", C.c_double), ("t", C.c_double),
+                ("calls", C.c_longlong), ("interactions", C.c_longlong),
+                ("ts_pairs", C.c_longlong), ("max_level", C.c_int), ("status", C.c_int)]
+
+
+class EnsembleDesc(C.Structure):
+    _fields_ = [("n_systems", C.c_int), ("offsets", C.c_void_p), ("pos", C.c_void_p),
+                ("vel", C.c_void_p), ("m", C.c_void_p), ("t_end", C.c_void_p),
+                ("dt_root", C.c_void_p), ("eta", C.c_double), ("eps", C.c_double),
+                ("max_level", C.c_int), ("max_calls", C.c_longlong),
+                ("wave_team_max_n", C.c_int), ("results", C.c_void_p),
+                ("timestep_mode", C.c_int)]
+
+
 class HoldStats(C.Structure):
     _fields_ = [("calls", C.c_longlong), ("interactions", C.c_longlong),
                 ("ts_pairs", C.c_longlong), ("max_level", C.c_int),
@@ -97,6 +112,8 @@ _SIGS = {
     "hhs_kick_get_config": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
     "hhs_timesteps_f64": (_i, [_P64, _i, _d, _d, _vp]),
     "hhs_timesteps_f32": (_i, [_P32, _i, _f, _f, _vp]),
+    "hhs_timesteps_fast_f64": (_i, [_P64, _i, _d, _d, _vp]),
+    "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
     "hhs_scan_workspace_bytes": (_sz, [_i]),
     "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
     "hhs_scan_workspace_thread": (_i, [_i, C.POINTER(_vp), C.POINTER(_sz)]),
@@ -123,6 +140,8 @@ _SIGS = {
     "hhs_hold_step_f64": (_i, [_vp, _P64, _i, _i, _d, _d, _d, _i, _vp]),
     "hhs_hold_f64": (_i, [_vp, _P64, _d, _d, _vp]),
     "hhs_ic_plummer": (_i, [_i, C.c_uint64, _i, _i, _d, _vp, _vp, _vp]),
+    "hhs_ensemble_workspace_bytes": (_sz, [_i, C.c_longlong]),
+    "hhs_ensemble_run_f64": (_i, [C.POINTER(EnsembleDesc), _vp, _vp, _sz, _vp]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -246,9 +265,16 @@ def kick_get_config(n_sink: int, n_src: int):
     return s.value, w.value
 
 
-def timesteps(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, stream=None):
+TIMESTEP_EXACT = 0
+TIMESTEP_FAST = 1
+
+
+def timesteps(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, stream=None, fast=False):
     n = sys.n if n is None else n
     s = sys.struct()
+    if fast:
+        check(_lib.hhs_timesteps_fast_f64(C.byref(s), n, eta, eps2, _stream(stream)))
+        return
     check(getattr(_lib, "hhs_timesteps_" + _sfx(sys))(C.byref(s), n, eta, eps2, _stream(stream)))
 
 
@@ -313,10 +339,13 @@ def energy(pos, vel, m, eps2=0.0, n=None, stream=None):
 class Globals:
     """nbodysystem_globals<double> + per-level buffers (hhs_globals)."""
 
-    def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48):
+    def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48,
+                 fast_timesteps: bool = False):
         h = C.c_void_p()
         check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
         self.h = h
+        if fast_timesteps:
+            check(_lib.hhs_globals_set_timestep_mode(h, TIMESTEP_FAST))
 
     def close(self):
         if self.h:
@@ -369,3 +398,64 @@ def ic_plummer(n: int, seed: int, mass_model: int = 0, virialize: bool = True,
     check(_lib.hhs_ic_plummer(n, seed, mass_model, 1 if virialize else 0, eps2,
                               pos.ctypes.data, vel.ctypes.data, m.ctypes.data))
     return pos, vel, m
+
+
+RESULT_DTYPE = [("e0", "f8"), ("e1", "f8"), ("t", "f8"), ("calls", "i8"),
+                ("interactions", "i8"), ("ts_pairs", "i8"), ("max_level", "i4"),
+                ("status", "i4")]
+
+
+class Ensemble:
+    """A batch of independent systems integrated by one persistent launch
+    (hhs_ensemble_run_f64).  `systems` is a list of (pos[n,3], vel[n,3],
+    m[n]) host arrays; t_end and dt_root are scalars or per-system
+    sequences."""
+
+    def __init__(self, systems, t_end, dt_root, eta, eps, max_level=0, max_calls=0,
+                 wave_team_max_n=0, fast_timesteps=False):
+        import numpy as np
+        import torch
+        self.n_systems = len(systems)
+        sizes = np.array([len(s[2]) for s in systems], dtype=np.int64)
+        self.offsets = np.zeros(self.n_systems + 1, dtype=np.int32)
+        np.cumsum(sizes, out=self.offsets[1:])
+        ntot = int(self.offsets[-1])
+        pos = np.zeros((max(ntot, 1), 3))
+        vel = np.zeros((max(ntot, 1), 3))
+        m = np.zeros(max(ntot, 1))
+        for k, (p, v, mm) in enumerate(systems):
+            a, b = self.offsets[k], self.offsets[k + 1]
+            pos[a:b], vel[a:b], m[a:b] = p, v, mm
+        dev = "cuda"
+        self.pos = torch.as_tensor(pos).to(dev)
+        self.vel = torch.as_tensor(vel).to(dev)
+        self.m = torch.as_tensor(m).to(dev)
+        self.off_dev = torch.as_tensor(self.offsets).to(dev)
+        self.t_end = torch.as_tensor(np.broadcast_to(np.asarray(t_end, float),
+                                                     (self.n_systems,)).copy()).to(dev)
+        self.dt_root = torch.as_tensor(np.broadcast_to(np.asarray(dt_root, float),
+                                                       (self.n_systems,)).copy()).to(dev)
+        self.results_dev = torch.zeros(max(self.n_systems, 1) * C.sizeof(SystemResult),
+                                       dtype=torch.uint8, device=dev)
+        self.ws_bytes = int(_lib.hhs_ensemble_workspace_bytes(self.n_systems, ntot))
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.desc = EnsembleDesc(self.n_systems, self.off_dev.data_ptr(), self.pos.data_ptr(),
+                                 self.vel.data_ptr(), self.m.data_ptr(), self.t_end.data_ptr(),
+                                 self.dt_root.data_ptr(), eta, eps, max_level, max_calls,
+                                 wave_team_max_n, self.results_dev.data_ptr(),
+                                 TIMESTEP_FAST if fast_timesteps else TIMESTEP_EXACT)
+
+    def run(self, stream=None):
+        check(_lib.hhs_ensemble_run_f64(C.byref(self.desc),
+                                        self.offsets.ctypes.data_as(C.c_void_p),
+                                        C.c_void_p(self.ws.data_ptr()), self.ws_bytes,
+                                        _stream(stream)))
+
+    def results(self):
+        import numpy as np
+        raw = self.results_dev.cpu().numpy().tobytes()
+        return np.frombuffer(raw, dtype=np.dtype(RESULT_DTYPE))[:self.n_systems].copy()
+
+    def state(self, k):
+        a, b = int(self.offsets[k]), int(self.offsets[k + 1])
+        return self.pos[a:b].cpu().numpy(), self.vel[a:b].cpu().numpy()

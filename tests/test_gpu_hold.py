@@ -42,6 +42,21 @@ def test_hold_one_root_step_matches_oracle(hhs, oracle, torch_cuda, n):
     assert np.array_equal(d.m.cpu().numpy(), host.m)
 
 
+def test_hold_fast_timesteps_matches_oracle(hhs, oracle, torch_cuda):
+    """HOLD with FAST timesteps: same recursion and trajectories to 1e-6."""
+    n, eps, eta = 256, 1e-2, 0.01
+    pos, vel, m = plummer(hhs, n, seed=4242, eps2=eps * eps)
+    host = oracle.HostParticles(n, pos, vel, m)
+    rc, ost = oracle.hold(host, 1.0 / 16, 1.0 / 32, eta, eps * eps)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(n, eta, eps, fast_timesteps=True)
+    g.hold(d, 1.0 / 16, 1.0 / 32)
+    torch_cuda.cuda.synchronize()
+    assert g.stats()["calls"] == ost.calls
+    p = d.pos.cpu().numpy()
+    assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
+
+
 def test_hold_energy_error_matches_fp64_run(hhs, oracle, torch_cuda):
     n, eps, eta = 256, 1e-2, 0.1
     pos, vel, m = plummer(hhs, n, seed=77, eps2=eps * eps)

@@ -54,6 +54,40 @@ def test_timesteps_circular_pair_closed_form(hhs, torch_cuda):
     assert np.allclose(got, min(tff, tfb), rtol=1e-14)
 
 
+@pytest.mark.parametrize("n", [2, 65, 1024, 3000])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_timesteps_fast_within_few_ulp(hhs, oracle, torch_cuda, n, eps2):
+    """FAST mode (rsqrt + Newton, fractions compared by cross-multiplication)
+    against the IEEE oracle: relative difference <= 1e-14 (a few ulp)."""
+    pos, vel, m = plummer(hhs, n, seed=90 + n, mass_model=1, eps2=eps2)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=0.01, eps2=eps2, fast=True)
+    torch_cuda.cuda.synchronize()
+    ref = oracle.timesteps(pos, vel, m, 0.01, eps2)
+    got = d.timestep.cpu().numpy()
+    assert np.max(np.abs(got / ref - 1.0)) <= 1e-14
+
+
+def test_timesteps_fast_edge_cases(hhs, oracle, torch_cuda):
+    # single particle: start value; cold start: fly-by terms never win;
+    # coincident pair with eps = 0: tau = 0 exactly (as the IEEE form)
+    d = hhs.DeviceParticles.from_host(np.zeros((1, 3)), np.zeros((1, 3)), np.ones(1))
+    hhs.timesteps(d, eta=0.01, fast=True)
+    torch_cuda.cuda.synchronize()
+    assert d.timestep.cpu().numpy()[0] == 1e30
+    pos, _, m = plummer(hhs, 100, seed=5)
+    vel = np.zeros((100, 3))
+    pos[7] = pos[3]
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=0.02, fast=True)
+    torch_cuda.cuda.synchronize()
+    ref = oracle.timesteps(pos, vel, m, 0.02)
+    got = d.timestep.cpu().numpy()
+    assert got[3] == 0.0 and got[7] == 0.0 and ref[3] == 0.0
+    ok = ref > 0
+    assert np.max(np.abs(got[ok] / ref[ok] - 1.0)) <= 1e-14
+
+
 def test_drift_bit_exact(hhs, oracle, torch_cuda):
     torch = torch_cuda
     n = 100003
