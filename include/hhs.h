@@ -294,6 +294,58 @@ size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles);
 int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, void *ws,
                          size_t ws_bytes, void *stream);
 
+/* ----------------------------------------------------------- serving -- */
+/* Continuous-batching request server (no reference counterpart).  A
+ * background scheduler admits requests in arrival order while their device
+ * footprint fits hbm_budget_bytes; systems of at most small_max_n particles
+ * run in ensemble batches (one persistent launch per batch, up to
+ * batch_streams batches in flight), larger ones run hhs_hold_f64 on one of
+ * big_workers streams.  Requests that arrive while others run are admitted
+ * as memory and slots free up, and their work overlaps on the device.  Each
+ * request's result depends only on its own data. */
+typedef struct hhs_server hhs_server;
+
+typedef struct hhs_server_config {
+    double eta, eps;
+    size_t hbm_budget_bytes;
+    int small_max_n;    /* 0 selects 2048                                   */
+    int max_n;          /* largest accepted system, 0 selects 65536         */
+    int max_batch;      /* systems per ensemble batch, 0 selects 4096       */
+    int batch_streams;  /* ensemble batches in flight, 0 selects 2          */
+    int big_workers;    /* concurrent large systems, 0 selects 2            */
+    int max_level;      /* 0 selects 40                                     */
+    int timestep_mode;  /* HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST          */
+} hhs_server_config;
+
+typedef struct hhs_request_info {
+    double t_submit, t_admit, t_start, t_done; /* seconds on the server clock */
+    hhs_system_result result;
+    int path;  /* 0: ensemble batch, 1: per-system HOLD                     */
+    int state; /* 0 pending, 1 admitted, 2 running, 3 done, 4 failed        */
+} hhs_request_info;
+
+typedef struct hhs_server_stats_t {
+    double now;
+    size_t used_bytes, peak_bytes;
+    int pending, running_batches, running_big;
+} hhs_server_stats_t;
+
+int hhs_server_create(hhs_server **out, const hhs_server_config *cfg);
+/* Finishes every submitted request, then stops. */
+int hhs_server_destroy(hhs_server *s);
+/* Copies the host arrays (pos, vel: 3n; m: n); HHS_ERR_BUDGET if the
+ * request alone exceeds the budget. */
+int hhs_server_submit(hhs_server *s, int n, const double *pos, const double *vel,
+                      const double *m, double t_end, double dt_root, long long *ticket);
+int hhs_server_poll(hhs_server *s, long long ticket, hhs_request_info *info);
+/* Blocks until the request is done; returns its status. */
+int hhs_server_wait(hhs_server *s, long long ticket, hhs_request_info *info);
+/* Final pos/vel (original particle order) of a finished request. */
+int hhs_server_fetch(hhs_server *s, long long ticket, double *pos_out, double *vel_out);
+/* Forget a finished request. */
+int hhs_server_release(hhs_server *s, long long ticket);
+int hhs_server_stats(hhs_server *s, hhs_server_stats_t *out);
+
 /* ------------------------------------------- initial conditions (host) -- */
 /* Host-only (no GPU use).  Plummer sphere in Henon units (G = M = 1,
  * E = -1/4; Aarseth, Henon & Wielen 1974), SplitMix64 stream `seed`,

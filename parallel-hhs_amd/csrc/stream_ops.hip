@@ -165,7 +165,10 @@ int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n,
     HHS_REQUIRE(pos && vel && m, "energy: null array");
     hipStream_t st = as_stream(stream);
     double *buf = nullptr;
-    HHS_HIP(hipMalloc(&buf, (2 * static_cast<size_t>(n) + 2) * sizeof(double)));
+    // stream-ordered allocation: no device-wide synchronisation while other
+    // streams are busy
+    HHS_HIP(hipMallocAsync(reinterpret_cast<void **>(&buf),
+                           (2 * static_cast<size_t>(n) + 2) * sizeof(double), st));
     double *row = buf, *kin = buf + n, *out = buf + 2 * static_cast<size_t>(n);
     int rc = HHS_OK;
     hipLaunchKernelGGL(energy_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pos, vel, m, n,
@@ -179,7 +182,7 @@ int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n,
     if (e == hipSuccess) e = hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = hip_status(e, "energy", __FILE__, __LINE__);
-    (void)hipFree(buf);
+    (void)hipFreeAsync(buf, st);
     if (rc) return rc;
     *kinetic = host[0];
     *potential = host[1];

@@ -81,6 +81,24 @@ class EnsembleDesc(C.Structure):
                 ("timestep_mode", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("eta", C.c_double), ("eps", C.c_double), ("hbm_budget_bytes", C.c_size_t),
+                ("small_max_n", C.c_int), ("max_n", C.c_int), ("max_batch", C.c_int),
+                ("batch_streams", C.c_int), ("big_workers", C.c_int), ("max_level", C.c_int),
+                ("timestep_mode", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("result", SystemResult), ("path", C.c_int),
+                ("state", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("now", C.c_double), ("used_bytes", C.c_size_t), ("peak_bytes", C.c_size_t),
+                ("pending", C.c_int), ("running_batches", C.c_int), ("running_big", C.c_int)]
+
+
 class HoldStats(C.Structure):
     _fields_ = [("calls", C.c_longlong), ("interactions", C.c_longlong),
                 ("ts_pairs", C.c_longlong), ("max_level", C.c_int),
@@ -142,6 +160,14 @@ _SIGS = {
     "hhs_ic_plummer": (_i, [_i, C.c_uint64, _i, _i, _d, _vp, _vp, _vp]),
     "hhs_ensemble_workspace_bytes": (_sz, [_i, C.c_longlong]),
     "hhs_ensemble_run_f64": (_i, [C.POINTER(EnsembleDesc), _vp, _vp, _sz, _vp]),
+    "hhs_server_create": (_i, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
+    "hhs_server_destroy": (_i, [_vp]),
+    "hhs_server_submit": (_i, [_vp, _i, _vp, _vp, _vp, _d, _d, C.POINTER(C.c_longlong)]),
+    "hhs_server_poll": (_i, [_vp, C.c_longlong, C.POINTER(RequestInfo)]),
+    "hhs_server_wait": (_i, [_vp, C.c_longlong, C.POINTER(RequestInfo)]),
+    "hhs_server_fetch": (_i, [_vp, C.c_longlong, _vp, _vp]),
+    "hhs_server_release": (_i, [_vp, C.c_longlong]),
+    "hhs_server_stats": (_i, [_vp, C.POINTER(ServerStats)]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -459,3 +485,80 @@ class Ensemble:
     def state(self, k):
         a, b = int(self.offsets[k]), int(self.offsets[k + 1])
         return self.pos[a:b].cpu().numpy(), self.vel[a:b].cpu().numpy()
+
+
+class Server:
+    """Continuous-batching HOLD server (hhs_server_*).  submit() copies the
+    host arrays and returns a ticket; wait()/poll() report timings on the
+    server clock; fetch() returns the final (pos, vel)."""
+
+    def __init__(self, eta, eps, hbm_budget_bytes=8 << 30, small_max_n=0, max_n=0,
+                 max_batch=0, batch_streams=0, big_workers=0, max_level=0,
+                 fast_timesteps=False):
+        cfg = ServerConfig(eta, eps, hbm_budget_bytes, small_max_n, max_n, max_batch,
+                           batch_streams, big_workers, max_level,
+                           TIMESTEP_FAST if fast_timesteps else TIMESTEP_EXACT)
+        h = C.c_void_p()
+        check(_lib.hhs_server_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            check(_lib.hhs_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, pos, vel, m, t_end, dt_root) -> int:
+        import numpy as np
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        t = C.c_longlong()
+        check(_lib.hhs_server_submit(self.h, len(m), pos.ctypes.data, vel.ctypes.data,
+                                     m.ctypes.data, t_end, dt_root, C.byref(t)))
+        self.sizes[t.value] = len(m)
+        return t.value
+
+    @staticmethod
+    def _info(ri):
+        r = ri.result
+        return {"t_submit": ri.t_submit, "t_admit": ri.t_admit, "t_start": ri.t_start,
+                "t_done": ri.t_done, "path": ri.path, "state": ri.state,
+                "result": {"e0": r.e0, "e1": r.e1, "t": r.t, "calls": r.calls,
+                           "interactions": r.interactions, "ts_pairs": r.ts_pairs,
+                           "max_level": r.max_level, "status": r.status}}
+
+    def poll(self, ticket: int) -> dict:
+        ri = RequestInfo()
+        check(_lib.hhs_server_poll(self.h, ticket, C.byref(ri)))
+        return self._info(ri)
+
+    def wait(self, ticket: int) -> dict:
+        ri = RequestInfo()
+        _lib.hhs_server_wait(self.h, ticket, C.byref(ri))
+        return self._info(ri)
+
+    def fetch(self, ticket: int):
+        import numpy as np
+        n = self.sizes[ticket]
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        check(_lib.hhs_server_fetch(self.h, ticket, pos.ctypes.data, vel.ctypes.data))
+        return pos, vel
+
+    def release(self, ticket: int):
+        check(_lib.hhs_server_release(self.h, ticket))
+        self.sizes.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(_lib.hhs_server_stats(self.h, C.byref(st)))
+        return {"now": st.now, "used_bytes": st.used_bytes, "peak_bytes": st.peak_bytes,
+                "pending": st.pending, "running_batches": st.running_batches,
+                "running_big": st.running_big}
