@@ -6,8 +6,14 @@ Salpeter masses (eps = 1e-4, seed 3): every particle is a sink and a source,
 N^2 = 1.718e10 interactions, acc written and vel += acc * dt.  `value` is
 Ginteractions/s over the timed steps (inputs resident in HBM).  The roofline
 object prices the kick at 20 flops per interaction against the FP32 vector
-peak.  One HOLD root step (dt = 1/64) of the same system is timed beside it,
-and the FP64 CPU oracle is timed on a bounded sample of sinks on the host.
+peak.  Beside the headline line the JSON carries:
+  * hold_root_step -- one HOLD root step (dt = 1/64) of the same system;
+  * ensemble -- 4096 Plummer systems (N log-uniform in [8, 2048], t = 1) in
+    one persistent launch, against the plain one-system-at-a-time loop;
+  * served -- a Poisson request stream through the continuous-batching
+    server, p50/p99 time-to-solution;
+  * cpu_baseline -- the FP64 oracle direct sum on the host, one thread, on
+    a bounded sample of sinks.
 """
 import argparse
 import json
@@ -23,9 +29,14 @@ sys.path.insert(0, os.path.join(ROOT, "parallel-hhs_amd"))
 METRIC = "Ginteractions/s at N=128k; ensemble systems/s + p99 latency at |dE/E|≤1e-6"
 N = 131072
 EPS = 1e-4
+ETA = 0.01
 SEED = 3
 FLOPS_PER_INTERACTION = 20
 FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def load_traffic():
@@ -60,15 +71,126 @@ def cpu_baseline(pos, m, acc_gpu, eps2, seconds=12.0):
     ref = oracle.accel_subset(pos, idx, pos, m, eps2)
     dt = time.perf_counter() - t0
     # precision of the benchmarked kick on the same sample
-    scale = np.empty(k)
+    scale = np.empty(256)
     for q, i in enumerate(idx[:256]):
         d = pos - pos[i]
         r2 = (d * d).sum(1) + eps2
         scale[q] = (m / r2).sum()
-    err = np.linalg.norm(acc_gpu[idx[:256]] - ref[:256], axis=1) / scale[:256]
-    return {"value": k * N / dt / 1e9, "unit": "Ginteractions/s", "cores": 1, "kind": "port",
-            "sample": f"{k} random sinks x {N} sources, FP64 oracle direct sum (oracle/hhs_oracle.c)",
+    err = np.linalg.norm(acc_gpu[idx[:256]] - ref[:256], axis=1) / scale
+    return {"value": round(k * N / dt / 1e9, 4), "unit": "Ginteractions/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{k} random sinks x {N} sources, FP64 oracle direct sum "
+                      "(oracle/hhs_oracle.c), one host thread",
             "seconds": round(dt, 3)}, float(err.max())
+
+
+def hold_bench(hhs, torch, pos, vel, m, dt):
+    d2 = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(N, ETA, EPS, fast_timesteps=True)
+    ke0, pe0 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    g.hold_step(d2, dt)
+    torch.cuda.synchronize()
+    hs = time.perf_counter() - h0
+    ke1, pe1 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
+    st = g.stats()
+    g.close()
+    return {"dt": dt, "timesteps": "fast", "seconds": round(hs, 4), "calls": st["calls"],
+            "levels": st["max_level"] + 1, "interactions": st["interactions"],
+            "ts_pairs": st["ts_pairs"],
+            "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0))}
+
+
+def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
+    """Configuration 4: 4096 Plummer systems, N log-uniform in [8, 2048],
+    equal masses, eta = 0.01, t = 1, eps = 1e-4 (unspecified there; stated
+    here), root steps of 1/16, FAST timesteps, one persistent launch."""
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(8), np.log(2048), n_systems)).astype(int)
+    systems = [hhs.ic_plummer(int(n), 4000 + k, 0, True, EPS * EPS)
+               for k, n in enumerate(sizes)]
+    t_end, dtr = 1.0, 1.0 / 16
+    ens = hhs.Ensemble(systems, t_end, dtr, ETA, EPS, fast_timesteps=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    r = ens.results()
+    de = np.abs((r["e1"] - r["e0"]) / r["e0"])
+    ok = (r["status"] == 0)
+    # the plain loop: the same systems one at a time through hhs_hold_f64
+    pick = np.argsort(sizes)[np.linspace(0, n_systems - 1, loop_sample).astype(int)]
+    loop_t = []
+    for k in pick:
+        p, v, mm = systems[k]
+        dd = hhs.DeviceParticles.from_host(p, v, mm)
+        g = hhs.Globals(len(mm), ETA, EPS, fast_timesteps=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.hold(dd, t_end, dtr)
+        torch.cuda.synchronize()
+        loop_t.append(time.perf_counter() - t0)
+        g.close()
+    loop_per_system = float(np.mean(loop_t))
+    sps = n_systems / wall
+    return {"systems": n_systems, "n_range": [8, 2048], "t_end": t_end, "dt_root": dtr,
+            "eta": ETA, "eps": EPS, "timesteps": "fast", "seconds": round(wall, 3),
+            "systems_per_s": round(sps, 2),
+            "systems_per_s_at_dE_le_1e-6": round(float(np.sum(ok & (de <= 1e-6))) / wall, 2),
+            "frac_dE_le_1e-6": round(float(np.mean(de <= 1e-6)), 4),
+            "dE_over_E": {"median": float(np.median(de)), "p99": float(np.quantile(de, 0.99)),
+                          "max": float(de.max())},
+            "status_ok": int(ok.sum()),
+            "interactions": int(r["interactions"].sum()), "ts_pairs": int(r["ts_pairs"].sum()),
+            "plain_loop": {"systems_sampled": loop_sample,
+                           "sample": "evenly spaced in sorted N (stratified)",
+                           "seconds_per_system": round(loop_per_system, 4),
+                           "systems_per_s": round(1.0 / loop_per_system, 3)},
+            "speedup_vs_plain_loop": round(sps * loop_per_system, 1)}
+
+
+def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
+    """Configuration 5 shape: Poisson arrivals of Plummer integrations with N
+    log-uniform in [n_lo, n_hi], end time uniform in [0.1, 1], eta = 0.01,
+    through the continuous-batching server; time-to-solution = completion -
+    arrival (server clock)."""
+    out = []
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n_req):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        p, v, m = hhs.ic_plummer(n, 9000 + k, 0, True, EPS * EPS)
+        reqs.append((p, v, m, float(rng.uniform(0.1, 1.0))))
+    for rate in rates:
+        gaps = np.random.default_rng(seed + int(rate * 1000)).exponential(1.0 / rate, n_req)
+        with hhs.Server(ETA, EPS, hbm_budget_bytes=64 << 30, small_max_n=2048, max_n=n_hi,
+                        batch_streams=2, big_workers=2, fast_timesteps=True) as srv:
+            tickets = []
+            t0 = time.perf_counter()
+            t_next = t0
+            for (p, v, m, te), gap in zip(reqs, gaps):
+                t_next += gap
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit(p, v, m, te, 1.0 / 16))
+            infos = [srv.wait(t) for t in tickets]
+            wall = time.perf_counter() - t0
+            peak = srv.stats()["peak_bytes"]
+        tts = np.array([i["t_done"] - i["t_submit"] for i in infos])
+        de = np.array([abs((i["result"]["e1"] - i["result"]["e0"]) / i["result"]["e0"])
+                       for i in infos])
+        out.append({"rate_per_s": rate, "requests": n_req,
+                    "ok": int(sum(i["state"] == 3 for i in infos)),
+                    "p50_s": round(float(np.median(tts)), 4),
+                    "p99_s": round(float(np.quantile(tts, 0.99)), 4),
+                    "throughput_req_per_s": round(n_req / wall, 3),
+                    "wall_s": round(wall, 3), "peak_admitted_bytes": int(peak),
+                    "frac_dE_le_1e-6": round(float(np.mean(de <= 1e-6)), 4)})
+    return {"n_range": [n_lo, n_hi], "t_end_range": [0.1, 1.0], "dt_root": 1.0 / 16,
+            "eta": ETA, "eps": EPS, "timesteps": "fast", "sweep": out}
 
 
 def main():
@@ -78,6 +200,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-hold", action="store_true", help="skip the HOLD root-step timing")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the built-in kick heuristic")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="served stream over the full N range [64, 65536] (long)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the kick shards by sinks "
@@ -96,12 +223,18 @@ def main():
     stream = torch.cuda.current_stream()
     dt = 1.0 / 64
 
+    tune = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        s_t, w_t, ms_t = hhs.autotune_kick(N, 3)
+        tune = {"sinks_per_lane": s_t, "waves_per_group": w_t, "ms": round(ms_t, 4),
+                "sweep_seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotune: {tune}")
     for _ in range(args.warmup):
         hhs.kick_self(d, dt=dt, fac=1.0, eps2=eps2)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     w0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
@@ -114,6 +247,7 @@ def main():
     gint = pairs / (ms * 1e-3) / 1e9
     achieved = FLOPS_PER_INTERACTION * pairs / (ms * 1e-3) / 1e12
     s_cfg, w_cfg = hhs.kick_get_config(N, N)
+    log(f"kick: {ms:.3f} ms, {gint:.1f} Ginteractions/s")
 
     out = {
         "metric": METRIC,
@@ -131,40 +265,32 @@ def main():
         "config": {"workload": "direct-sum kick, Plummer N=131072 Salpeter masses, eps=1e-4, "
                                "seed 3 (FP64 positions, FP64-derived FP32 separations, "
                                "two-sum compensated FP32 sums)",
-                   "n": N, "eps": EPS, "sinks_per_lane": s_cfg, "waves_per_group": w_cfg},
+                   "n": N, "eps": EPS, "sinks_per_lane": s_cfg, "waves_per_group": w_cfg,
+                   "autotune": tune},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": load_traffic(),
                      "flops_per_interaction": FLOPS_PER_INTERACTION,
-                     "kernel": "kick_kernel<double,S,W,true>"},
+                     "kernel": "kick_kernel<double,S,W,0,true>"},
         "wall_s_timed": round(wall, 4),
         "ic_seconds": round(t_ic, 2),
     }
-
     acc_gpu = d.acc.cpu().numpy()
     if not args.no_hold:
-        # One HOLD root step of the same system (timesteps, splits, kicks,
-        # drifts, scatter-back at every level).
-        d2 = hhs.DeviceParticles.from_host(pos, vel, m)
-        g = hhs.Globals(N, 0.01, EPS)
-        ke0, pe0 = hhs.energy(d2.pos, d2.vel, d2.m, eps2)
-        torch.cuda.synchronize()
-        h0 = time.perf_counter()
-        g.hold_step(d2, dt)
-        torch.cuda.synchronize()
-        hs = time.perf_counter() - h0
-        ke1, pe1 = hhs.energy(d2.pos, d2.vel, d2.m, eps2)
-        st = g.stats()
-        out["hold_root_step"] = {
-            "dt": dt, "seconds": round(hs, 4), "calls": st["calls"],
-            "levels": st["max_level"] + 1, "interactions": st["interactions"],
-            "ts_pairs": st["ts_pairs"],
-            "kick_ginteractions_per_s_effective": round(st["interactions"] / hs / 1e9, 2),
-            "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0)),
-        }
-        g.close()
-
+        out["hold_root_step"] = hold_bench(hhs, torch, pos, vel, m, dt)
+        log(f"hold: {out['hold_root_step']}")
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(hhs, torch)
+        log(f"ensemble: {out['ensemble']}")
+    if not args.no_serve:
+        if args.serve_full:
+            out["served"] = serve_bench(hhs, [0.05, 0.2], 32, 64, 65536)
+        else:
+            out["served"] = serve_bench(hhs, [4.0, 16.0], 48, 64, 4096)
+            out["served"]["note"] = ("N range capped at 4096 for the default run; "
+                                     "--serve-full runs [64, 65536]")
+        log(f"served: {out['served']}")
     if not args.no_cpu:
         cb, ferr = cpu_baseline(pos, m, acc_gpu, eps2)
         out["cpu_baseline"] = cb

@@ -123,6 +123,33 @@ int hhs_kick_cross_f32(const hhs_particles_f32 *so, const hhs_particles_f32 *si,
  * tile (1,2,4,8).  0/0 restores the built-in heuristic.  Applies to calls
  * issued afterwards from any thread. */
 int hhs_kick_set_config(int sinks_per_lane, int waves_per_group);
+
+/* Kick arithmetic for FP64 particle sets (FP32 sets always use MIXED32):
+ * MIXED32:   x_j - x_i in FP64 rounded to FP32; r^2, rsqrt, m r^-3 and the
+ *            sum in FP32 with two-sum compensation (15 VALU ops/interaction)
+ * F64_RSQ32: separations, r^2 and the sum in FP64; rsqrt and m r^-3 in FP32
+ *            (14 ops)
+ * F64:       FP64 throughout with the hardware 1/sqrt estimate (relative
+ *            error <= 2^-23) (12 ops)
+ * All three meet the 1e-6 normalised force bar. */
+#define HHS_KICK_MIXED32 0
+#define HHS_KICK_F64_RSQ32 1
+#define HHS_KICK_F64 2
+int hhs_kick_set_precision(int mode);
+int hhs_kick_get_precision(void);
+
+/* Kick autotuner (no reference counterpart).  Times every (sinks per lane,
+ * waves per group) configuration on a synthetic n-particle FP64 system
+ * (reps timed launches each, default stream, blocking), installs the
+ * fastest for sink counts in n's power-of-two size class, and appends a
+ * line "kick <class> <S> <W> <ms>" to cache_path when it is non-NULL.
+ * hhs_autotune_load installs the entries of such a file (later lines
+ * win); hhs_autotune_clear returns to the built-in heuristic.  An explicit
+ * hhs_kick_set_config overrides tuned entries. */
+int hhs_autotune_kick(int n, int reps, const char *cache_path, int *best_s, int *best_w,
+                      double *best_ms);
+int hhs_autotune_load(const char *cache_path);
+int hhs_autotune_clear(void);
 int hhs_kick_get_config(int n_sink, int n_src, int *sinks_per_lane, int *waves_per_group);
 
 /* --------------------------------------------------------- timesteps -- */

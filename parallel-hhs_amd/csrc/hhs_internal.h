@@ -69,7 +69,7 @@ template <typename T> struct KickArgs {
     const T *s1_pos, *s1_m;
     int n1;
     T h;          // velocity update factor: vel += acc * h   (h = fac * dt)
-    float eps2;
+    double eps2;
 };
 
 struct KickConfig {

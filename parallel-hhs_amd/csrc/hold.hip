@@ -136,13 +136,13 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
         // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow
         KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
                             lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
-                            lb.fast.m,   nf,         dt,          static_cast<float>(g->eps2)};
+                            lb.fast.m,   nf,         dt,          g->eps2};
         if ((rc = launch_kick<double>(ks, st))) return rc;
         S.interactions += static_cast<long long>(ns) * (ns + nf);
         if (nf > 0) {
             KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
                                 lb.slow.m,   ns,          nullptr,     nullptr, 0,
-                                dt,          static_cast<float>(g->eps2)};
+                                dt,          g->eps2};
             if ((rc = launch_kick<double>(kf, st))) return rc;
             S.interactions += static_cast<long long>(nf) * ns;
         }

@@ -20,6 +20,9 @@
 //    Knuth's two-sum, so the rounding error of the long sum stays at the
 //    level of one block.
 #include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <vector>
 
 #include "hhs_internal.h"
 #include "pair_math.h"
@@ -29,43 +32,110 @@ namespace {
 
 constexpr int kFold = 64;  // sources per plain-FP32 block before a two-sum fold
 
-template <typename T, int S, bool SOFT>
-__device__ __forceinline__ void accumulate_range(const T *__restrict__ sp,
-                                                 const T *__restrict__ sm, int jb, int je,
-                                                 const T (&xi)[S], const T (&yi)[S],
-                                                 const T (&zi)[S], float eps2, float (&hx)[S],
-                                                 float (&lx)[S], float (&hy)[S], float (&ly)[S],
-                                                 float (&hz)[S], float (&lz)[S]) {
-    for (int j0 = jb; j0 < je; j0 += kFold) {
-        const int j1 = min(j0 + kFold, je);
-        float px[S], py[S], pz[S];
+// Per-sink force accumulators for the three precision modes.
+//  P = 0: separations x_j - x_i in FP64 rounded once to FP32; r^2, rsqrt,
+//         m r^-3 and the sum in FP32, summed plainly over blocks of kFold
+//         sources and folded into a (hi, lo) two-sum accumulator.
+//         15 VALU ops + 1 v_rsq_f32 per interaction (FP64 storage).
+//  P = 1: separations, r^2 and the sum in FP64; v_rsq_f32 and the weight
+//         m r^-3 in FP32.  14 VALU ops + 1 v_rsq_f32.
+//  P = 2: FP64 throughout with the hardware reciprocal-square-root estimate
+//         v_rsq_f64 (relative error <= 2^-23, FP32-level); 12 VALU ops +
+//         1 v_rsq_f64.  The weight carries ~3 x 2^-23 relative error per
+//         pair term, inside the 1e-6 force bar.
+template <typename T, int S, int P, bool SOFT> struct KickAcc;
+
+template <typename T, int S, bool SOFT> struct KickAcc<T, S, 0, SOFT> {
+    float hx[S], lx[S], hy[S], ly[S], hz[S], lz[S];
+    __device__ KickAcc() {
 #pragma unroll
-        for (int s = 0; s < S; ++s) px[s] = py[s] = pz[s] = 0.0f;
+        for (int s = 0; s < S; ++s) hx[s] = lx[s] = hy[s] = ly[s] = hz[s] = lz[s] = 0.0f;
+    }
+    __device__ void range(const T *__restrict__ sp, const T *__restrict__ sm, int jb, int je,
+                          const T (&xi)[S], const T (&yi)[S], const T (&zi)[S], float eps2f,
+                          double) {
+        for (int j0 = jb; j0 < je; j0 += kFold) {
+            const int j1 = min(j0 + kFold, je);
+            float px[S], py[S], pz[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) px[s] = py[s] = pz[s] = 0.0f;
 #pragma unroll 4
-        for (int j = j0; j < j1; ++j) {
-            const T xj = sp[3 * j + 0];
-            const T yj = sp[3 * j + 1];
-            const T zj = sp[3 * j + 2];
-            const float mj = static_cast<float>(sm[j]);
+            for (int j = j0; j < j1; ++j) {
+                const T xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
+                const float mj = static_cast<float>(sm[j]);
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-                pair_accel<SOFT>(xj, yj, zj, mj, xi[s], yi[s], zi[s], eps2, px[s], py[s], pz[s]);
-        }
+                for (int s = 0; s < S; ++s)
+                    pair_accel<SOFT>(xj, yj, zj, mj, xi[s], yi[s], zi[s], eps2f, px[s], py[s],
+                                     pz[s]);
+            }
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            two_sum_add(hx[s], lx[s], px[s]);
-            two_sum_add(hy[s], ly[s], py[s]);
-            two_sum_add(hz[s], lz[s], pz[s]);
+            for (int s = 0; s < S; ++s) {
+                two_sum_add(hx[s], lx[s], px[s]);
+                two_sum_add(hy[s], ly[s], py[s]);
+                two_sum_add(hz[s], lz[s], pz[s]);
+            }
         }
     }
-}
+    __device__ void total(int s, double &ax, double &ay, double &az) const {
+        ax = static_cast<double>(hx[s]) + static_cast<double>(lx[s]);
+        ay = static_cast<double>(hy[s]) + static_cast<double>(ly[s]);
+        az = static_cast<double>(hz[s]) + static_cast<double>(lz[s]);
+    }
+};
 
-template <typename T, int S, int W, bool SOFT>
+template <int S, int P, bool SOFT> struct KickAcc64 {
+    double ax[S], ay[S], az[S];
+    __device__ KickAcc64() {
+#pragma unroll
+        for (int s = 0; s < S; ++s) ax[s] = ay[s] = az[s] = 0.0;
+    }
+    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm, int jb,
+                          int je, const double (&xi)[S], const double (&yi)[S],
+                          const double (&zi)[S], float, double eps2) {
+#pragma unroll 4
+        for (int j = jb; j < je; ++j) {
+            const double xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
+            const double mj = sm[j];
+            const float mjf = static_cast<float>(mj);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const double dx = xj - xi[s], dy = yj - yi[s], dz = zj - zi[s];
+                const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+                double w;
+                if constexpr (P == 1) {
+                    const float rinv = __builtin_amdgcn_rsqf(static_cast<float>(r2));
+                    w = static_cast<double>((mjf * rinv) * (rinv * rinv));
+                } else {
+                    const double rinv = __builtin_amdgcn_rsq(r2);
+                    w = (mj * rinv) * (rinv * rinv);
+                }
+                if (!SOFT) w = (r2 > 0.0) ? w : 0.0;  // src/kick.cuh:25 (skip d2 == 0)
+                ax[s] = __builtin_fma(w, dx, ax[s]);
+                ay[s] = __builtin_fma(w, dy, ay[s]);
+                az[s] = __builtin_fma(w, dz, az[s]);
+            }
+        }
+    }
+    __device__ void total(int s, double &x, double &y, double &z) const {
+        x = ax[s];
+        y = ay[s];
+        z = az[s];
+    }
+};
+
+template <int S, bool SOFT> struct KickAcc<double, S, 1, SOFT> : KickAcc64<S, 1, SOFT> {};
+template <int S, bool SOFT> struct KickAcc<double, S, 2, SOFT> : KickAcc64<S, 2, SOFT> {};
+
+// One workgroup: 64*S sinks (S per lane), W waves splitting the source list
+// (segment 0 then segment 1), per-wave totals merged through LDS in wave
+// order (deterministic), epilogue acc <- a, vel += a * h.
+template <typename T, int S, int W, int P, bool SOFT>
 __global__ __launch_bounds__(64 * W) void kick_kernel(
     const T *__restrict__ si_pos, T *__restrict__ si_vel, T *__restrict__ si_acc, int n_sink,
     const T *__restrict__ s0_pos, const T *__restrict__ s0_m, int n0,
-    const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1, T h, float eps2) {
-    __shared__ float red[(W > 1 ? W - 1 : 1) * 6 * S * 64];
+    const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1, T h, float eps2f,
+    double eps2) {
+    __shared__ double red[(W > 1 ? W - 1 : 1) * 3 * S * 64];
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -80,54 +150,37 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
         yi[s] = si_pos[3 * i + 1];
         zi[s] = si_pos[3 * i + 2];
     }
-    float hx[S], lx[S], hy[S], ly[S], hz[S], lz[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) hx[s] = lx[s] = hy[s] = ly[s] = hz[s] = lz[s] = 0.0f;
-
+    KickAcc<T, S, P, SOFT> acc;
     // This wave's contiguous share of the concatenated source list.
     const int total = n0 + n1;
     const int chunk = (total + W - 1) / W;
     const int jb = min(wave * chunk, total);
     const int je = min(jb + chunk, total);
-    if (jb < n0)
-        accumulate_range<T, S, SOFT>(s0_pos, s0_m, jb, min(je, n0), xi, yi, zi, eps2, hx, lx, hy,
-                                     ly, hz, lz);
-    if (je > n0)
-        accumulate_range<T, S, SOFT>(s1_pos, s1_m, max(jb, n0) - n0, je - n0, xi, yi, zi, eps2,
-                                     hx, lx, hy, ly, hz, lz);
+    if (jb < n0) acc.range(s0_pos, s0_m, jb, min(je, n0), xi, yi, zi, eps2f, eps2);
+    if (je > n0) acc.range(s1_pos, s1_m, max(jb, n0) - n0, je - n0, xi, yi, zi, eps2f, eps2);
 
     double ax[S], ay[S], az[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        ax[s] = static_cast<double>(hx[s]) + static_cast<double>(lx[s]);
-        ay[s] = static_cast<double>(hy[s]) + static_cast<double>(ly[s]);
-        az[s] = static_cast<double>(hz[s]) + static_cast<double>(lz[s]);
-    }
+    for (int s = 0; s < S; ++s) acc.total(s, ax[s], ay[s], az[s]);
     if constexpr (W > 1) {
         if (wave > 0) {
-            float *r = red + (wave - 1) * 6 * S * 64;
+            double *r = red + (wave - 1) * 3 * S * 64;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                r[(0 * S + s) * 64 + lane] = hx[s];
-                r[(1 * S + s) * 64 + lane] = lx[s];
-                r[(2 * S + s) * 64 + lane] = hy[s];
-                r[(3 * S + s) * 64 + lane] = ly[s];
-                r[(4 * S + s) * 64 + lane] = hz[s];
-                r[(5 * S + s) * 64 + lane] = lz[s];
+                r[(0 * S + s) * 64 + lane] = ax[s];
+                r[(1 * S + s) * 64 + lane] = ay[s];
+                r[(2 * S + s) * 64 + lane] = az[s];
             }
         }
         __syncthreads();
         if (wave != 0) return;
         for (int w = 1; w < W; ++w) {  // fixed order: deterministic
-            const float *r = red + (w - 1) * 6 * S * 64;
+            const double *r = red + (w - 1) * 3 * S * 64;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                ax[s] += static_cast<double>(r[(0 * S + s) * 64 + lane]) +
-                         static_cast<double>(r[(1 * S + s) * 64 + lane]);
-                ay[s] += static_cast<double>(r[(2 * S + s) * 64 + lane]) +
-                         static_cast<double>(r[(3 * S + s) * 64 + lane]);
-                az[s] += static_cast<double>(r[(4 * S + s) * 64 + lane]) +
-                         static_cast<double>(r[(5 * S + s) * 64 + lane]);
+                ax[s] += r[(0 * S + s) * 64 + lane];
+                ay[s] += r[(1 * S + s) * 64 + lane];
+                az[s] += r[(2 * S + s) * 64 + lane];
             }
         }
     }
@@ -150,52 +203,76 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     }
 }
 
-template <typename T, int S, int W>
-int launch_sw(const KickArgs<T> &a, hipStream_t st) {
+template <typename T, int S, int W, int P>
+int launch_swp(const KickArgs<T> &a, hipStream_t st) {
     const int tile = 64 * S;
     dim3 grid((a.n_sink + tile - 1) / tile), block(64 * W);
-    if (a.eps2 > 0.0f)
-        hipLaunchKernelGGL((kick_kernel<T, S, W, true>), grid, block, 0, st, a.si_pos, a.si_vel,
-                           a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1,
-                           a.h, a.eps2);
+    const double eps2 = a.eps2;
+    const float eps2f = static_cast<float>(a.eps2);
+    if (a.eps2 > 0.0)
+        hipLaunchKernelGGL((kick_kernel<T, S, W, P, true>), grid, block, 0, st, a.si_pos,
+                           a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
+                           a.n1, a.h, eps2f, eps2);
     else
-        hipLaunchKernelGGL((kick_kernel<T, S, W, false>), grid, block, 0, st, a.si_pos, a.si_vel,
-                           a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1,
-                           a.h, a.eps2);
+        hipLaunchKernelGGL((kick_kernel<T, S, W, P, false>), grid, block, 0, st, a.si_pos,
+                           a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
+                           a.n1, a.h, eps2f, eps2);
     HHS_LAUNCH_CHECK("kick_kernel");
     return HHS_OK;
 }
 
+template <typename T, int S, int W>
+int launch_sw(const KickArgs<T> &a, int P, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (P == 1) return launch_swp<T, S, W, 1>(a, st);
+        if (P == 2) return launch_swp<T, S, W, 2>(a, st);
+    }
+    return launch_swp<T, S, W, 0>(a, st);
+}
+
 template <typename T, int S>
-int launch_s(const KickArgs<T> &a, int W, hipStream_t st) {
+int launch_s(const KickArgs<T> &a, int W, int P, hipStream_t st) {
     switch (W) {
-    case 1: return launch_sw<T, S, 1>(a, st);
-    case 2: return launch_sw<T, S, 2>(a, st);
-    case 4: return launch_sw<T, S, 4>(a, st);
-    case 8: return launch_sw<T, S, 8>(a, st);
+    case 1: return launch_sw<T, S, 1>(a, P, st);
+    case 2: return launch_sw<T, S, 2>(a, P, st);
+    case 4: return launch_sw<T, S, 4>(a, P, st);
+    case 8: return launch_sw<T, S, 8>(a, P, st);
     }
     set_error("kick: unsupported waves per group %d", W);
     return HHS_ERR_INVALID;
 }
 
 std::atomic<int> g_force_s{0}, g_force_w{0};
+std::atomic<int> g_precision{HHS_KICK_MIXED32};
 
 bool valid_s(int s) { return s == 1 || s == 2 || s == 4; }
 bool valid_w(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
 
+// Autotuned configurations per sink-count size class (floor(log2 n)),
+// encoded S << 8 | W; 0 = use the heuristic.
+std::atomic<int> g_tuned[32];
+int size_class(int n) { return n <= 1 ? 0 : 31 - __builtin_clz(static_cast<unsigned>(n)); }
+
 }  // namespace
 
-// Heuristic: enough sinks per lane to amortise each scalar source fetch,
-// then split sources across waves until the chip holds ~2 waves per SIMD
-// (256 CUs x 4 SIMDs), keeping >= 64 sources per wave.
+// Heuristic from the MI355X sweep at N = 131072 (FP64 storage): W = 8 with
+// S = 1 or 2 reaches 2606 Ginteractions/s (~15 VALU ops per interaction at
+// the full issue rate), S = 4 / W = 4 2164, S = 4 / W = 8 2012.  Latency is
+// hidden by waves, not by deeper register blocking: use up to 8 waves per
+// workgroup (8 waves per SIMD at N = 131072) while each wave keeps >= 64
+// sources, and 2 sinks per lane only for large sink counts.
 KickConfig kick_pick_config(int n_sink, int n_src) {
     KickConfig c;
     const int fs = g_force_s.load(), fw = g_force_w.load();
-    c.sinks_per_lane = n_sink >= 65536 ? 4 : (n_sink >= 8192 ? 2 : 1);
-    const int tiles = (n_sink + 64 * c.sinks_per_lane - 1) / (64 * c.sinks_per_lane);
+    c.sinks_per_lane = n_sink >= 65536 ? 2 : 1;
     int w = 1;
-    while (w < 8 && tiles * w < 2048 && n_src / (2 * w) >= 64) w *= 2;
+    while (w < 8 && n_src / (2 * w) >= 64) w *= 2;
     c.waves = w;
+    const int tuned = g_tuned[size_class(n_sink)].load();
+    if (tuned) {
+        c.sinks_per_lane = tuned >> 8;
+        c.waves = tuned & 0xff;
+    }
     if (valid_s(fs)) c.sinks_per_lane = fs;
     if (valid_w(fw)) c.waves = fw;
     return c;
@@ -208,10 +285,11 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     HHS_REQUIRE(a.n0 == 0 || (a.s0_pos && a.s0_m), "kick: null source segment 0");
     HHS_REQUIRE(a.n1 == 0 || (a.s1_pos && a.s1_m), "kick: null source segment 1");
     const KickConfig c = kick_pick_config(a.n_sink, a.n0 + a.n1);
+    const int P = g_precision.load();
     switch (c.sinks_per_lane) {
-    case 1: return launch_s<T, 1>(a, c.waves, st);
-    case 2: return launch_s<T, 2>(a, c.waves, st);
-    case 4: return launch_s<T, 4>(a, c.waves, st);
+    case 1: return launch_s<T, 1>(a, c.waves, P, st);
+    case 2: return launch_s<T, 2>(a, c.waves, P, st);
+    case 4: return launch_s<T, 4>(a, c.waves, P, st);
     }
     set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
     return HHS_ERR_INVALID;
@@ -229,7 +307,7 @@ static int kick_self(const P *sys, int n, T dt, T fac, T eps2, void *stream) {
     HHS_REQUIRE(sys, "kick_self: null system");
     HHS_REQUIRE(n >= 0 && n <= sys->n, "kick_self: n=%d outside [0, %d]", n, sys->n);
     KickArgs<T> a{sys->pos, sys->vel, sys->acc, n, sys->pos, sys->m, n, nullptr, nullptr, 0,
-                  fac * dt, static_cast<float>(eps2)};
+                  fac * dt, static_cast<double>(eps2)};
     return launch_kick<T>(a, as_stream(stream));
 }
 
@@ -242,7 +320,7 @@ static int kick_cross(const P *so, const P *si, int n_sink, int n_src, T dt, T f
     HHS_REQUIRE(n_src >= 0 && n_src <= so->n, "kick_cross: n_src=%d outside [0, %d]", n_src,
                 so->n);
     KickArgs<T> a{si->pos, si->vel, si->acc, n_sink, so->pos, so->m, n_src, nullptr, nullptr, 0,
-                  fac * dt, static_cast<float>(eps2)};
+                  fac * dt, static_cast<double>(eps2)};
     return launch_kick<T>(a, as_stream(stream));
 }
 
@@ -278,6 +356,108 @@ int hhs_kick_set_config(int sinks_per_lane, int waves_per_group) {
     HHS_REQUIRE(valid_w(waves_per_group), "kick config: waves_per_group must be 1, 2, 4 or 8");
     g_force_s.store(sinks_per_lane);
     g_force_w.store(waves_per_group);
+    return HHS_OK;
+}
+
+int hhs_kick_set_precision(int mode) {
+    HHS_REQUIRE(mode == HHS_KICK_MIXED32 || mode == HHS_KICK_F64_RSQ32 || mode == HHS_KICK_F64,
+                "kick precision: mode must be HHS_KICK_MIXED32, HHS_KICK_F64_RSQ32 or HHS_KICK_F64");
+    g_precision.store(mode);
+    return HHS_OK;
+}
+
+int hhs_kick_get_precision(void) { return g_precision.load(); }
+
+int hhs_autotune_kick(int n, int reps, const char *cache_path, int *best_s, int *best_w,
+                      double *best_ms) {
+    HHS_REQUIRE(n >= 64, "autotune_kick: n must be >= 64");
+    if (reps <= 0) reps = 3;
+    std::vector<double> p(3 * static_cast<size_t>(n)), v(3 * static_cast<size_t>(n)),
+        m(static_cast<size_t>(n));
+    int rc = hhs_ic_plummer(n, 12345, 0, 0, 0.0, p.data(), v.data(), m.data());
+    if (rc) return rc;
+    double *dev = nullptr;
+    const size_t nn = static_cast<size_t>(n);
+    HHS_HIP(hipMalloc(&dev, nn * 80));  // pos, vel, acc (3n each) and m
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemcpy(dev, p.data(), nn * 24, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dev + 3 * nn, v.data(), nn * 24, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dev + 9 * nn, m.data(), nn * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const int saved_s = g_force_s.load(), saved_w = g_force_w.load();
+    double best = 1e300;
+    int bs = 0, bw = 0;
+    for (int s : {1, 2, 4}) {
+        for (int w : {1, 2, 4, 8}) {
+            if (e != hipSuccess || rc) break;
+            g_force_s.store(s);
+            g_force_w.store(w);
+            KickArgs<double> a{dev,      dev + 3 * nn, dev + 6 * nn, n,   dev, dev + 9 * nn,
+                               n,        nullptr,      nullptr,      0,   0.0, 1e-8};
+            rc = launch_kick<double>(a, nullptr);  // warm-up
+            if (!rc) e = hipEventRecord(e0, nullptr);
+            for (int r = 0; r < reps && !rc && e == hipSuccess; ++r)
+                rc = launch_kick<double>(a, nullptr);
+            if (!rc && e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (!rc && e == hipSuccess && ms / reps < best) {
+                best = ms / reps;
+                bs = s;
+                bw = w;
+            }
+        }
+    }
+    g_force_s.store(saved_s);
+    g_force_w.store(saved_w);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_status(e, "autotune_kick", __FILE__, __LINE__);
+    if (rc) return rc;
+    const int cls = size_class(n);
+    g_tuned[cls].store(bs << 8 | bw);
+    if (cache_path) {
+        FILE *f = std::fopen(cache_path, "a");
+        if (!f) {
+            set_error("autotune_kick: cannot append to %s", cache_path);
+            return HHS_ERR_INVALID;
+        }
+        std::fprintf(f, "kick %d %d %d %.6f\n", cls, bs, bw, best);
+        std::fclose(f);
+    }
+    if (best_s) *best_s = bs;
+    if (best_w) *best_w = bw;
+    if (best_ms) *best_ms = best;
+    return HHS_OK;
+}
+
+int hhs_autotune_load(const char *cache_path) {
+    HHS_REQUIRE(cache_path, "autotune_load: null path");
+    FILE *f = std::fopen(cache_path, "r");
+    if (!f) {
+        set_error("autotune_load: cannot open %s", cache_path);
+        return HHS_ERR_INVALID;
+    }
+    char kind[16];
+    int cls, s, w;
+    double ms;
+    int loaded = 0;
+    while (std::fscanf(f, "%15s %d %d %d %lf", kind, &cls, &s, &w, &ms) == 5) {
+        if (std::strcmp(kind, "kick") == 0 && cls >= 0 && cls < 32 && valid_s(s) && valid_w(w)) {
+            g_tuned[cls].store(s << 8 | w);  // later lines win
+            ++loaded;
+        }
+    }
+    std::fclose(f);
+    return loaded > 0 ? HHS_OK : (set_error("autotune_load: no entries in %s", cache_path),
+                                  HHS_ERR_INVALID);
+}
+
+int hhs_autotune_clear(void) {
+    for (auto &t : g_tuned) t.store(0);
     return HHS_OK;
 }
 

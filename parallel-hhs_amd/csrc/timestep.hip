@@ -20,46 +20,106 @@
 namespace hhs {
 namespace {
 
-template <typename T>
-__global__ __launch_bounds__(256) void timesteps_kernel(const T *__restrict__ pos,
-                                                        const T *__restrict__ vel,
-                                                        const T *__restrict__ m,
-                                                        T *__restrict__ ts, int n, T eta,
-                                                        T eps2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Exact (IEEE) or fast evaluation, folded into a running minimum.  The
+// exact form keeps minval; the fast form keeps the fraction (num, den).
+template <typename T, bool FAST> struct TsAcc {
+    T num = TsConst<T>::huge, den = T(1);
+    __device__ void pair(T xi, T yi, T zi, T vxi, T vyi, T vzi, T mi, const T *pos, const T *vel,
+                         const T *m, int j, T eta, T c_ff, T eps2, bool other) {
+        if constexpr (FAST)
+            pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
+                               pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j],
+                               eta, c_ff, eps2, other, num, den);
+        else
+            pair_timestep<T>(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
+                             pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta,
+                             c_ff, eps2, other, num);
+    }
+    __device__ void merge(T onum, T oden) {
+        if constexpr (FAST)
+            fold_fraction(onum, oden, true, num, den);
+        else if (onum < num)
+            num = onum;
+    }
+    __device__ T value() const {
+        if constexpr (FAST)
+            return num / den;
+        else
+            return num;
+    }
+};
+
+// 64 sinks per workgroup, one per lane of every wave; the W waves split the
+// source range and their minima are merged through LDS in wave order.  The
+// minimum does not depend on the order, so the exact form stays bit-exact.
+template <typename T, int W, bool FAST>
+__global__ __launch_bounds__(64 * W) void timesteps_kernel(const T *__restrict__ pos,
+                                                           const T *__restrict__ vel,
+                                                           const T *__restrict__ m,
+                                                           T *__restrict__ ts, int n, T eta,
+                                                           T eps2) {
+    __shared__ T red[2][(W > 1 ? W - 1 : 1) * 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = blockIdx.x * 64 + lane;
     const int ic = i < n ? i : n - 1;
     const T xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
     const T vxi = vel[3 * ic], vyi = vel[3 * ic + 1], vzi = vel[3 * ic + 2];
     const T mi = m[ic];
     const T c_ff = eta / TsConst<T>::sqrt2;
-    T minval = TsConst<T>::huge;
-    for (int j = 0; j < n; ++j)
-        pair_timestep<T>(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
-                         vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta, c_ff, eps2, j != i,
-                         minval);
-    if (i < n) ts[i] = minval;
+    const int chunk = (n + W - 1) / W;
+    const int jb = min(wave * chunk, n), je = min(jb + chunk, n);
+    TsAcc<T, FAST> acc;
+    for (int j = jb; j < je; ++j)
+        acc.pair(xi, yi, zi, vxi, vyi, vzi, mi, pos, vel, m, j, eta, c_ff, eps2, j != i);
+    if constexpr (W > 1) {
+        if (wave > 0) {
+            red[0][(wave - 1) * 64 + lane] = acc.num;
+            red[1][(wave - 1) * 64 + lane] = acc.den;
+        }
+        __syncthreads();
+        if (wave != 0) return;
+        for (int w = 1; w < W; ++w) acc.merge(red[0][(w - 1) * 64 + lane], red[1][(w - 1) * 64 + lane]);
+    }
+    if (i < n) ts[i] = acc.value();
 }
 
-// Fast FP64 form (pair_timestep_fast): rsqrt + Newton instead of IEEE
-// sqrt/divide, one division per sink.  Agrees with timesteps_kernel to a
-// few ulp; not bit-exact.
-__global__ __launch_bounds__(256) void timesteps_fast_kernel(const double *__restrict__ pos,
-                                                             const double *__restrict__ vel,
-                                                             const double *__restrict__ m,
-                                                             double *__restrict__ ts, int n,
-                                                             double eta, double eps2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ic = i < n ? i : n - 1;
-    const double xi = pos[3 * ic], yi = pos[3 * ic + 1], zi = pos[3 * ic + 2];
-    const double vxi = vel[3 * ic], vyi = vel[3 * ic + 1], vzi = vel[3 * ic + 2];
-    const double mi = m[ic];
-    const double c_ff = eta / TsConst<double>::sqrt2;
-    double bnum = TsConst<double>::huge, bden = 1.0;
-    for (int j = 0; j < n; ++j)
-        pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
-                           pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta,
-                           c_ff, eps2, j != i, bnum, bden);
-    if (i < n) ts[i] = bnum / bden;
+// Waves per workgroup: up to 8 waves per SIMD over the chip (2048 SIMDs x 4
+// at 256 CUs) while every wave keeps >= 64 sources.
+int pick_waves(int n) {
+    const int tiles = (n + 63) / 64;
+    int w = 1;
+    while (w < 8 && tiles * w < 8192 && n / (2 * w) >= 64) w *= 2;
+    return w;
+}
+
+template <typename T, bool FAST>
+int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
+              hipStream_t s) {
+    HHS_REQUIRE(n >= 0, "timesteps: negative n");
+    if (n == 0) return HHS_OK;
+    HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
+    const dim3 grid((n + 63) / 64);
+    switch (pick_waves(n)) {
+    case 8:
+        hipLaunchKernelGGL((timesteps_kernel<T, 8, FAST>), grid, dim3(512), 0, s, pos, vel, m, ts,
+                           n, eta, eps2);
+        break;
+    case 4:
+        hipLaunchKernelGGL((timesteps_kernel<T, 4, FAST>), grid, dim3(256), 0, s, pos, vel, m, ts,
+                           n, eta, eps2);
+        break;
+    case 2:
+        hipLaunchKernelGGL((timesteps_kernel<T, 2, FAST>), grid, dim3(128), 0, s, pos, vel, m, ts,
+                           n, eta, eps2);
+        break;
+    default:
+        hipLaunchKernelGGL((timesteps_kernel<T, 1, FAST>), grid, dim3(64), 0, s, pos, vel, m, ts,
+                           n, eta, eps2);
+        break;
+    }
+    HHS_LAUNCH_CHECK("timesteps_kernel");
+    return HHS_OK;
 }
 
 }  // namespace
@@ -67,24 +127,15 @@ __global__ __launch_bounds__(256) void timesteps_fast_kernel(const double *__res
 template <typename T>
 int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
                      hipStream_t s) {
-    HHS_REQUIRE(n >= 0, "timesteps: negative n");
-    if (n == 0) return HHS_OK;
-    HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
-    dim3 block(256), grid((n + 255) / 256);
-    hipLaunchKernelGGL((timesteps_kernel<T>), grid, block, 0, s, pos, vel, m, ts, n, eta, eps2);
-    HHS_LAUNCH_CHECK("timesteps_kernel");
-    return HHS_OK;
+    return launch_ts<T, false>(pos, vel, m, ts, n, eta, eps2, s);
 }
 
+// Fast FP64 form (pair_timestep_fast): rsqrt + Newton instead of IEEE
+// sqrt/divide, one division per sink.  Agrees with the exact form to a few
+// ulp; not bit-exact.
 int launch_timesteps_fast(const double *pos, const double *vel, const double *m, double *ts,
                           int n, double eta, double eps2, hipStream_t s) {
-    HHS_REQUIRE(n >= 0, "timesteps: negative n");
-    if (n == 0) return HHS_OK;
-    HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
-    dim3 block(256), grid((n + 255) / 256);
-    hipLaunchKernelGGL(timesteps_fast_kernel, grid, block, 0, s, pos, vel, m, ts, n, eta, eps2);
-    HHS_LAUNCH_CHECK("timesteps_fast_kernel");
-    return HHS_OK;
+    return launch_ts<double, true>(pos, vel, m, ts, n, eta, eps2, s);
 }
 
 template int launch_timesteps<double>(const double *, const double *, const double *, double *,

@@ -127,6 +127,12 @@ _SIGS = {
     "hhs_kick_cross_f64": (_i, [_P64, _P64, _i, _i, _d, _d, _d, _vp]),
     "hhs_kick_cross_f32": (_i, [_P32, _P32, _i, _i, _f, _f, _f, _vp]),
     "hhs_kick_set_config": (_i, [_i, _i]),
+    "hhs_kick_set_precision": (_i, [_i]),
+    "hhs_kick_get_precision": (_i, []),
+    "hhs_autotune_kick": (_i, [_i, _i, C.c_char_p, C.POINTER(_i), C.POINTER(_i),
+                               C.POINTER(_d)]),
+    "hhs_autotune_load": (_i, [C.c_char_p]),
+    "hhs_autotune_clear": (_i, []),
     "hhs_kick_get_config": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
     "hhs_timesteps_f64": (_i, [_P64, _i, _d, _d, _vp]),
     "hhs_timesteps_f32": (_i, [_P32, _i, _f, _f, _vp]),
@@ -283,6 +289,22 @@ def kick_cross(so: DeviceParticles, si: DeviceParticles, n_sink=None, n_src=None
 
 def kick_set_config(sinks_per_lane: int, waves: int):
     check(_lib.hhs_kick_set_config(sinks_per_lane, waves))
+
+
+def autotune_kick(n: int, reps: int = 3, cache_path: str | None = None):
+    """Sweep the kick configurations at size n; returns (S, W, ms)."""
+    s, w, ms = C.c_int(), C.c_int(), C.c_double()
+    path = cache_path.encode() if cache_path else None
+    check(_lib.hhs_autotune_kick(n, reps, path, C.byref(s), C.byref(w), C.byref(ms)))
+    return s.value, w.value, ms.value
+
+
+def autotune_load(cache_path: str):
+    check(_lib.hhs_autotune_load(cache_path.encode()))
+
+
+def autotune_clear():
+    check(_lib.hhs_autotune_clear())
 
 
 def kick_get_config(n_sink: int, n_src: int):
@@ -562,3 +584,14 @@ class Server:
         return {"now": st.now, "used_bytes": st.used_bytes, "peak_bytes": st.peak_bytes,
                 "pending": st.pending, "running_batches": st.running_batches,
                 "running_big": st.running_big}
+
+
+KICK_MIXED32, KICK_F64_RSQ32, KICK_F64 = 0, 1, 2
+
+
+def kick_set_precision(mode: int):
+    check(_lib.hhs_kick_set_precision(mode))
+
+
+def kick_get_precision() -> int:
+    return int(_lib.hhs_kick_get_precision())

@@ -71,13 +71,13 @@ def test_invalid_arguments_return_status(hhs):
 
 def test_kick_config_heuristic(hhs):
     s, w = hhs.kick_get_config(131072, 131072)
-    assert (s, w) == (4, 4)
+    assert (s, w) == (2, 8)
     s, w = hhs.kick_get_config(100, 100)
     assert s == 1 and 1 <= w <= 8
-    hhs.kick_set_config(2, 8)
-    assert hhs.kick_get_config(131072, 131072) == (2, 8)
+    hhs.kick_set_config(4, 2)
+    assert hhs.kick_get_config(131072, 131072) == (4, 2)
     hhs.kick_set_config(0, 0)
-    assert hhs.kick_get_config(131072, 131072) == (4, 4)
+    assert hhs.kick_get_config(131072, 131072) == (2, 8)
 
 
 # ------------------------------------------------------ initial conditions
@@ -116,3 +116,24 @@ def test_salpeter_masses(hhs):
     hist, edges = np.histogram(np.log10(m / m.min() * 0.1), bins=np.linspace(-1, 1, 9))
     slope = np.polyfit(0.5 * (edges[1:] + edges[:-1]), np.log10(hist), 1)[0]
     assert slope == pytest.approx(-1.35, abs=0.1)
+
+
+def test_autotune_cache_round_trip(hhs, tmp_path):
+    """The tuning cache is plain text; loading it installs per-size-class
+    kick configurations (host logic only, no GPU)."""
+    path = tmp_path / "tune.txt"
+    path.write_text("kick 17 4 2 1.0\nkick 12 1 8 0.5\nkick 17 2 4 0.9\ngarbage line\n")
+    try:
+        hhs.autotune_load(str(path))
+        assert hhs.kick_get_config(131072, 131072) == (2, 4)   # class 17, later line wins
+        assert hhs.kick_get_config(4096, 4096) == (1, 8)       # class 12
+        hhs.kick_set_config(4, 1)                              # explicit setting overrides
+        assert hhs.kick_get_config(131072, 131072) == (4, 1)
+        hhs.kick_set_config(0, 0)
+    finally:
+        hhs.autotune_clear()
+    assert hhs.kick_get_config(131072, 131072) == (2, 8)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("nothing here\n")
+    with pytest.raises(hhs.HHSError):
+        hhs.autotune_load(str(bad))

@@ -60,6 +60,25 @@ def test_kick_every_config(hhs, oracle, torch_cuda, s, w):
     assert err < TOL, f"S={s} W={w}: {err:.3e}"
 
 
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_kick_precision_modes(hhs, oracle, torch_cuda, precision, eps2):
+    """MIXED32 (0), F64_RSQ32 (1) and F64 (2) all meet the force bar."""
+    n = 3001
+    pos, vel, m = plummer(hhs, n, seed=55, mass_model=1, eps2=eps2)
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_set_precision(precision)
+    try:
+        hhs.kick_self(d, dt=0.5, fac=1.0, eps2=eps2)
+        torch_cuda.cuda.synchronize()
+    finally:
+        hhs.kick_set_precision(0)
+    vel_ref, acc_ref = oracle.kick(pos, vel, pos, m, 0.5, 1.0, eps2)
+    err = _check(d.acc.cpu().numpy(), acc_ref, force_scale(pos, m, eps2))
+    assert err < TOL, f"precision {precision}: {err:.3e}"
+    assert hhs.kick_get_precision() == 0
+
+
 @pytest.mark.parametrize("n_sink,n_src", [(1, 5), (300, 517), (517, 300), (64, 1), (129, 4096)])
 def test_kick_cross_matches_oracle(hhs, oracle, torch_cuda, n_sink, n_src):
     eps2 = 1e-8
@@ -146,3 +165,22 @@ def test_kick_f32_storage(hhs, oracle, torch_cuda):
     ref = oracle.accel(pos32, pos32, m.astype(np.float32).astype(np.float64), eps2)
     err = _check(d.acc.cpu().numpy().astype(np.float64), ref, force_scale(pos32, m, eps2))
     assert err < 1e-5  # FP32 storage: FP32 separations
+
+
+def test_autotune_installs_a_valid_config(hhs, oracle, torch_cuda, tmp_path):
+    path = tmp_path / "tune.txt"
+    try:
+        s, w, ms = hhs.autotune_kick(2048, 2, str(path))
+        assert s in (1, 2, 4) and w in (1, 2, 4, 8) and ms > 0
+        assert hhs.kick_get_config(2048, 2048) == (s, w)
+        line = path.read_text().split()
+        assert line[0] == "kick" and int(line[1]) == 11 and (int(line[2]), int(line[3])) == (s, w)
+        pos, vel, m = plummer(hhs, 2048, seed=3, eps2=1e-8)
+        d = _dev(hhs, pos, vel, m)
+        hhs.kick_self(d, eps2=1e-8)
+        torch_cuda.cuda.synchronize()
+        err = _check(d.acc.cpu().numpy(), oracle.accel(pos, pos, m, 1e-8),
+                     force_scale(pos, m, 1e-8))
+        assert err < TOL
+    finally:
+        hhs.autotune_clear()
