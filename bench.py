@@ -121,7 +121,8 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
     de = np.abs((r["e1"] - r["e0"]) / r["e0"])
     ok = (r["status"] == 0)
     # the plain loop: the same systems one at a time through hhs_hold_f64
-    pick = np.argsort(sizes)[np.linspace(0, n_systems - 1, loop_sample).astype(int)]
+    pick = (np.argsort(sizes)[np.linspace(0, n_systems - 1, loop_sample).astype(int)]
+            if loop_sample > 0 else [])
     loop_t = []
     for k in pick:
         p, v, mm = systems[k]
@@ -133,7 +134,7 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
         torch.cuda.synchronize()
         loop_t.append(time.perf_counter() - t0)
         g.close()
-    loop_per_system = float(np.mean(loop_t))
+    loop_per_system = float(np.mean(loop_t)) if loop_t else float("nan")
     sps = n_systems / wall
     return {"systems": n_systems, "n_range": [8, 2048], "t_end": t_end, "dt_root": dtr,
             "eta": ETA, "eps": EPS, "timesteps": "fast", "seconds": round(wall, 3),
@@ -147,8 +148,8 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
             "plain_loop": {"systems_sampled": loop_sample,
                            "sample": "evenly spaced in sorted N (stratified)",
                            "seconds_per_system": round(loop_per_system, 4),
-                           "systems_per_s": round(1.0 / loop_per_system, 3)},
-            "speedup_vs_plain_loop": round(sps * loop_per_system, 1)}
+                           "systems_per_s": round(1.0 / loop_per_system, 3) if loop_t else None},
+            "speedup_vs_plain_loop": round(sps * loop_per_system, 1) if loop_t else None}
 
 
 def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
@@ -203,8 +204,13 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in kick heuristic")
+    ap.add_argument("--loop-sample", type=int, default=12,
+                    help="systems timed through the plain per-system loop (0 skips it)")
     ap.add_argument("--serve-full", action="store_true",
                     help="served stream over the full N range [64, 65536] (long)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="first run one device-to-device copy of MB MiB (a kernel of known "
+                         "HBM traffic for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the kick shards by sinks "
@@ -215,6 +221,14 @@ def main():
     import hhs
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     eps2 = EPS * EPS
     t0 = time.perf_counter()
     pos, vel, m = hhs.ic_plummer(N, SEED, 1, True, eps2)
@@ -281,7 +295,7 @@ def main():
         out["hold_root_step"] = hold_bench(hhs, torch, pos, vel, m, dt)
         log(f"hold: {out['hold_root_step']}")
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_bench(hhs, torch)
+        out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample)
         log(f"ensemble: {out['ensemble']}")
     if not args.no_serve:
         if args.serve_full:
