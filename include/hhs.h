@@ -167,10 +167,15 @@ int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2
 #define HHS_TIMESTEP_EXACT 0 /* IEEE sqrt/divide, bit-exact with the formula   */
 #define HHS_TIMESTEP_FAST 1  /* rsqrt + Newton, one divide per particle; agrees
                                 with EXACT to a few ulp                         */
+#define HHS_TIMESTEP_FP32 2  /* FP64 differences rounded to FP32, criterion in
+                                FP32; relative error ~1e-7                      */
 /* Same criterion as hhs_timesteps_f64 in the FAST form (no counterpart in
  * the reference). */
 int hhs_timesteps_fast_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
                            void *stream);
+/* Same criterion in any of the three modes. */
+int hhs_timesteps_mode_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
+                           int mode, void *stream);
 
 /* ------------------------------------------------------ scan / split -- */
 /* Device workspace for the single-pass scan (tile counter, epoch, tile

@@ -67,7 +67,7 @@ struct EnsArgs {
     int n_small;
     unsigned int *counters;
     hhs_system_result *results;
-    int fast_ts;
+    int ts_mode;
 };
 
 template <int TEAM> struct Team;
@@ -170,7 +170,7 @@ __device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, const Team
         const double vxi = a.wvel[3 * ic], vyi = a.wvel[3 * ic + 1], vzi = a.wvel[3 * ic + 2];
         const double mi = a.wm[ic];
         double minval = TsConst<double>::huge;
-        if (a.fast_ts) {
+        if (a.ts_mode == HHS_TIMESTEP_FAST) {
             double bden = 1.0;
             for (int j = jb; j < je; ++j)
                 pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
@@ -178,6 +178,17 @@ __device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, const Team
                                    a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta, a.c_ff,
                                    a.eps2, j != i, minval, bden);
             minval = minval / bden;
+        } else if (a.ts_mode == HHS_TIMESTEP_FP32) {
+            float bnum = static_cast<float>(TsConst<double>::huge), bden = 1.0f;
+            const float mif = static_cast<float>(mi), etaf = static_cast<float>(a.eta),
+                        cff = static_cast<float>(a.c_ff);
+            for (int j = jb; j < je; ++j)
+                pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mif, a.wpos[3 * j],
+                                  a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
+                                  a.wvel[3 * j + 1], a.wvel[3 * j + 2],
+                                  static_cast<float>(a.wm[j]), etaf, cff, a.eps2f, j != i, bnum,
+                                  bden);
+            minval = static_cast<double>(bnum) / static_cast<double>(bden);
         } else {
             for (int j = jb; j < je; ++j)
                 pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
@@ -575,9 +586,10 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     a.n_small = static_cast<int>(small.size());
     a.counters = counters;
     a.results = d->results;
-    HHS_REQUIRE(d->timestep_mode == HHS_TIMESTEP_EXACT || d->timestep_mode == HHS_TIMESTEP_FAST,
+    HHS_REQUIRE(d->timestep_mode == HHS_TIMESTEP_EXACT || d->timestep_mode == HHS_TIMESTEP_FAST ||
+                    d->timestep_mode == HHS_TIMESTEP_FP32,
                 "ensemble: bad timestep_mode %d", d->timestep_mode);
-    a.fast_ts = d->timestep_mode == HHS_TIMESTEP_FAST;
+    a.ts_mode = d->timestep_mode;
 
     // Persistent grid: enough workgroups for every big system and every
     // group of four small ones, capped at 8 per CU.

@@ -84,8 +84,8 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t s);
 template <typename T>
 int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
                      hipStream_t s);
-int launch_timesteps_fast(const double *pos, const double *vel, const double *m, double *ts,
-                          int n, double eta, double eps2, hipStream_t s);
+int launch_timesteps_mode(int mode, const double *pos, const double *vel, const double *m,
+                          double *ts, int n, double eta, double eps2, hipStream_t s);
 
 // ----------------------------------------------------------- scan / split
 // Workspace layout: u32 header[4] = {tile counter, epoch, raw fast count, 0}

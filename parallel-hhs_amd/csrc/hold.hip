@@ -106,11 +106,8 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     int rc;
     // src/hold.cuh:74-76
     if (calc) {
-        rc = g->timestep_mode == HHS_TIMESTEP_FAST
-                 ? launch_timesteps_fast(total.pos, total.vel, total.m, total.timestep, n, g->eta,
-                                         g->eps2, st)
-                 : launch_timesteps<double>(total.pos, total.vel, total.m, total.timestep, n,
-                                            g->eta, g->eps2, st);
+        rc = launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m,
+                                   total.timestep, n, g->eta, g->eps2, st);
         if (rc) return rc;
         S.ts_pairs += static_cast<long long>(n) * (n - 1);
     }
@@ -225,8 +222,9 @@ double hhs_globals_eps(const hhs_globals *g) { return g ? g->eps : 0.0; }
 
 int hhs_globals_set_timestep_mode(hhs_globals *g, int mode) {
     HHS_REQUIRE(g, "set_timestep_mode: null globals");
-    HHS_REQUIRE(mode == HHS_TIMESTEP_EXACT || mode == HHS_TIMESTEP_FAST,
-                "set_timestep_mode: mode must be HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST");
+    HHS_REQUIRE(mode == HHS_TIMESTEP_EXACT || mode == HHS_TIMESTEP_FAST ||
+                    mode == HHS_TIMESTEP_FP32,
+                "set_timestep_mode: mode must be HHS_TIMESTEP_EXACT, _FAST or _FP32");
     g->timestep_mode = mode;
     return HHS_OK;
 }

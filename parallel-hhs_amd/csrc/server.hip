@@ -263,8 +263,7 @@ void worker_loop(hhs_server *s) {
     bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
               hhs_globals_create(&g, nmax, s->cfg.eta, s->cfg.eps, s->cfg.max_level) == HHS_OK &&
               hipMalloc(&dev, nn * 96 + 1024) == hipSuccess;
-    if (ok && s->cfg.timestep_mode == HHS_TIMESTEP_FAST)
-        ok = hhs_globals_set_timestep_mode(g, HHS_TIMESTEP_FAST) == HHS_OK;
+    if (ok) ok = hhs_globals_set_timestep_mode(g, s->cfg.timestep_mode) == HHS_OK;
     double *dpos = static_cast<double *>(dev);
     double *dvel = dpos + 3 * nn, *dacc = dvel + 3 * nn, *dts = dacc + 3 * nn, *dm = dts + nn;
     int *did = reinterpret_cast<int *>(dm + nn), *dpar = did + nn;
@@ -342,7 +341,8 @@ int hhs_server_create(hhs_server **out, const hhs_server_config *cfg) {
     *out = nullptr;
     HHS_REQUIRE(cfg->eta > 0.0 && cfg->eps >= 0.0, "server_create: need eta > 0, eps >= 0");
     HHS_REQUIRE(cfg->hbm_budget_bytes > 0, "server_create: zero HBM budget");
-    HHS_REQUIRE(cfg->timestep_mode == HHS_TIMESTEP_EXACT || cfg->timestep_mode == HHS_TIMESTEP_FAST,
+    HHS_REQUIRE(cfg->timestep_mode == HHS_TIMESTEP_EXACT || cfg->timestep_mode == HHS_TIMESTEP_FAST ||
+                    cfg->timestep_mode == HHS_TIMESTEP_FP32,
                 "server_create: bad timestep_mode");
     auto *s = new (std::nothrow) hhs_server();
     if (!s) return HHS_ERR_ALLOC;

@@ -14,51 +14,63 @@
 // The reference's early `return` before a barrier (src/timestep.cuh:70 vs
 // :89,:95) has no counterpart: there are no barriers here, and lanes past
 // n compute on a clamped sink and do not store.
+#include <type_traits>
+
 #include "hhs_internal.h"
 #include "pair_math.h"
 
 namespace hhs {
 namespace {
 
-// Exact (IEEE) or fast evaluation, folded into a running minimum.  The
-// exact form keeps minval; the fast form keeps the fraction (num, den).
-template <typename T, bool FAST> struct TsAcc {
-    T num = TsConst<T>::huge, den = T(1);
+// Running minimum for the three evaluation modes (HHS_TIMESTEP_EXACT,
+// _FAST, _FP32).  The exact form keeps minval; the others keep a fraction
+// (num, den) compared by cross-multiplication and divide once at the end.
+template <typename T, int MODE> struct TsAcc {
+    using V = typename std::conditional<MODE == HHS_TIMESTEP_FP32, float, T>::type;
+    V num = static_cast<V>(TsConst<T>::huge), den = V(1);
     __device__ void pair(T xi, T yi, T zi, T vxi, T vyi, T vzi, T mi, const T *pos, const T *vel,
                          const T *m, int j, T eta, T c_ff, T eps2, bool other) {
-        if constexpr (FAST)
+        if constexpr (MODE == HHS_TIMESTEP_FAST)
             pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
                                pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j],
                                eta, c_ff, eps2, other, num, den);
+        else if constexpr (MODE == HHS_TIMESTEP_FP32)
+            pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, static_cast<float>(mi), pos[3 * j],
+                              pos[3 * j + 1], pos[3 * j + 2], vel[3 * j], vel[3 * j + 1],
+                              vel[3 * j + 2], static_cast<float>(m[j]), static_cast<float>(eta),
+                              static_cast<float>(c_ff), static_cast<float>(eps2), other, num, den);
         else
             pair_timestep<T>(xi, yi, zi, vxi, vyi, vzi, mi, pos[3 * j], pos[3 * j + 1],
                              pos[3 * j + 2], vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], m[j], eta,
                              c_ff, eps2, other, num);
     }
-    __device__ void merge(T onum, T oden) {
-        if constexpr (FAST)
+    __device__ void merge(V onum, V oden) {
+        if constexpr (MODE == HHS_TIMESTEP_FAST)
             fold_fraction(onum, oden, true, num, den);
+        else if constexpr (MODE == HHS_TIMESTEP_FP32)
+            fold_fraction32(onum, oden, true, num, den);
         else if (onum < num)
             num = onum;
     }
     __device__ T value() const {
-        if constexpr (FAST)
-            return num / den;
-        else
+        if constexpr (MODE == HHS_TIMESTEP_EXACT)
             return num;
+        else
+            return static_cast<T>(num) / static_cast<T>(den);
     }
 };
 
 // 64 sinks per workgroup, one per lane of every wave; the W waves split the
 // source range and their minima are merged through LDS in wave order.  The
 // minimum does not depend on the order, so the exact form stays bit-exact.
-template <typename T, int W, bool FAST>
+template <typename T, int W, int MODE>
 __global__ __launch_bounds__(64 * W) void timesteps_kernel(const T *__restrict__ pos,
                                                            const T *__restrict__ vel,
                                                            const T *__restrict__ m,
                                                            T *__restrict__ ts, int n, T eta,
                                                            T eps2) {
-    __shared__ T red[2][(W > 1 ? W - 1 : 1) * 64];
+    using V = typename TsAcc<T, MODE>::V;
+    __shared__ V red[2][(W > 1 ? W - 1 : 1) * 64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = blockIdx.x * 64 + lane;
@@ -69,7 +81,7 @@ __global__ __launch_bounds__(64 * W) void timesteps_kernel(const T *__restrict__
     const T c_ff = eta / TsConst<T>::sqrt2;
     const int chunk = (n + W - 1) / W;
     const int jb = min(wave * chunk, n), je = min(jb + chunk, n);
-    TsAcc<T, FAST> acc;
+    TsAcc<T, MODE> acc;
     for (int j = jb; j < je; ++j)
         acc.pair(xi, yi, zi, vxi, vyi, vzi, mi, pos, vel, m, j, eta, c_ff, eps2, j != i);
     if constexpr (W > 1) {
@@ -93,7 +105,7 @@ int pick_waves(int n) {
     return w;
 }
 
-template <typename T, bool FAST>
+template <typename T, int MODE>
 int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
               hipStream_t s) {
     HHS_REQUIRE(n >= 0, "timesteps: negative n");
@@ -102,19 +114,19 @@ int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps
     const dim3 grid((n + 63) / 64);
     switch (pick_waves(n)) {
     case 8:
-        hipLaunchKernelGGL((timesteps_kernel<T, 8, FAST>), grid, dim3(512), 0, s, pos, vel, m, ts,
+        hipLaunchKernelGGL((timesteps_kernel<T, 8, MODE>), grid, dim3(512), 0, s, pos, vel, m, ts,
                            n, eta, eps2);
         break;
     case 4:
-        hipLaunchKernelGGL((timesteps_kernel<T, 4, FAST>), grid, dim3(256), 0, s, pos, vel, m, ts,
+        hipLaunchKernelGGL((timesteps_kernel<T, 4, MODE>), grid, dim3(256), 0, s, pos, vel, m, ts,
                            n, eta, eps2);
         break;
     case 2:
-        hipLaunchKernelGGL((timesteps_kernel<T, 2, FAST>), grid, dim3(128), 0, s, pos, vel, m, ts,
+        hipLaunchKernelGGL((timesteps_kernel<T, 2, MODE>), grid, dim3(128), 0, s, pos, vel, m, ts,
                            n, eta, eps2);
         break;
     default:
-        hipLaunchKernelGGL((timesteps_kernel<T, 1, FAST>), grid, dim3(64), 0, s, pos, vel, m, ts,
+        hipLaunchKernelGGL((timesteps_kernel<T, 1, MODE>), grid, dim3(64), 0, s, pos, vel, m, ts,
                            n, eta, eps2);
         break;
     }
@@ -127,15 +139,21 @@ int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps
 template <typename T>
 int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
                      hipStream_t s) {
-    return launch_ts<T, false>(pos, vel, m, ts, n, eta, eps2, s);
+    return launch_ts<T, HHS_TIMESTEP_EXACT>(pos, vel, m, ts, n, eta, eps2, s);
 }
 
 // Fast FP64 form (pair_timestep_fast): rsqrt + Newton instead of IEEE
 // sqrt/divide, one division per sink.  Agrees with the exact form to a few
 // ulp; not bit-exact.
-int launch_timesteps_fast(const double *pos, const double *vel, const double *m, double *ts,
-                          int n, double eta, double eps2, hipStream_t s) {
-    return launch_ts<double, true>(pos, vel, m, ts, n, eta, eps2, s);
+int launch_timesteps_mode(int mode, const double *pos, const double *vel, const double *m,
+                          double *ts, int n, double eta, double eps2, hipStream_t s) {
+    switch (mode) {
+    case HHS_TIMESTEP_EXACT: return launch_ts<double, HHS_TIMESTEP_EXACT>(pos, vel, m, ts, n, eta, eps2, s);
+    case HHS_TIMESTEP_FAST: return launch_ts<double, HHS_TIMESTEP_FAST>(pos, vel, m, ts, n, eta, eps2, s);
+    case HHS_TIMESTEP_FP32: return launch_ts<double, HHS_TIMESTEP_FP32>(pos, vel, m, ts, n, eta, eps2, s);
+    }
+    set_error("timesteps: unknown mode %d", mode);
+    return HHS_ERR_INVALID;
 }
 
 template int launch_timesteps<double>(const double *, const double *, const double *, double *,
@@ -161,7 +179,15 @@ int hhs_timesteps_fast_f64(const hhs_particles_f64 *sys, int n, double eta, doub
                            void *stream) {
     HHS_REQUIRE(sys, "timesteps: null system");
     HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
-    return launch_timesteps_fast(sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
+    return launch_timesteps_mode(HHS_TIMESTEP_FAST, sys->pos, sys->vel, sys->m, sys->timestep, n,
+                                 eta, eps2, as_stream(stream));
+}
+
+int hhs_timesteps_mode_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
+                           void *stream) {
+    HHS_REQUIRE(sys, "timesteps: null system");
+    HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
+    return launch_timesteps_mode(mode, sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
                                  as_stream(stream));
 }
 

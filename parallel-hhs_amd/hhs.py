@@ -137,6 +137,7 @@ _SIGS = {
     "hhs_timesteps_f64": (_i, [_P64, _i, _d, _d, _vp]),
     "hhs_timesteps_f32": (_i, [_P32, _i, _f, _f, _vp]),
     "hhs_timesteps_fast_f64": (_i, [_P64, _i, _d, _d, _vp]),
+    "hhs_timesteps_mode_f64": (_i, [_P64, _i, _d, _d, _i, _vp]),
     "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
     "hhs_scan_workspace_bytes": (_sz, [_i]),
     "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
@@ -315,13 +316,24 @@ def kick_get_config(n_sink: int, n_src: int):
 
 TIMESTEP_EXACT = 0
 TIMESTEP_FAST = 1
+TIMESTEP_FP32 = 2
+
+
+def _ts_mode(fast):
+    """fast: False/True or a mode constant (TIMESTEP_EXACT/_FAST/_FP32)."""
+    if fast is True:
+        return TIMESTEP_FAST
+    if fast is False or fast is None:
+        return TIMESTEP_EXACT
+    return int(fast)
 
 
 def timesteps(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, stream=None, fast=False):
     n = sys.n if n is None else n
     s = sys.struct()
-    if fast:
-        check(_lib.hhs_timesteps_fast_f64(C.byref(s), n, eta, eps2, _stream(stream)))
+    mode = _ts_mode(fast)
+    if mode != TIMESTEP_EXACT:
+        check(_lib.hhs_timesteps_mode_f64(C.byref(s), n, eta, eps2, mode, _stream(stream)))
         return
     check(getattr(_lib, "hhs_timesteps_" + _sfx(sys))(C.byref(s), n, eta, eps2, _stream(stream)))
 
@@ -392,8 +404,8 @@ class Globals:
         h = C.c_void_p()
         check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
         self.h = h
-        if fast_timesteps:
-            check(_lib.hhs_globals_set_timestep_mode(h, TIMESTEP_FAST))
+        if _ts_mode(fast_timesteps) != TIMESTEP_EXACT:
+            check(_lib.hhs_globals_set_timestep_mode(h, _ts_mode(fast_timesteps)))
 
     def close(self):
         if self.h:
@@ -491,7 +503,7 @@ class Ensemble:
                                  self.vel.data_ptr(), self.m.data_ptr(), self.t_end.data_ptr(),
                                  self.dt_root.data_ptr(), eta, eps, max_level, max_calls,
                                  wave_team_max_n, self.results_dev.data_ptr(),
-                                 TIMESTEP_FAST if fast_timesteps else TIMESTEP_EXACT)
+                                 _ts_mode(fast_timesteps))
 
     def run(self, stream=None):
         check(_lib.hhs_ensemble_run_f64(C.byref(self.desc),
@@ -518,8 +530,7 @@ class Server:
                  max_batch=0, batch_streams=0, big_workers=0, max_level=0,
                  fast_timesteps=False):
         cfg = ServerConfig(eta, eps, hbm_budget_bytes, small_max_n, max_n, max_batch,
-                           batch_streams, big_workers, max_level,
-                           TIMESTEP_FAST if fast_timesteps else TIMESTEP_EXACT)
+                           batch_streams, big_workers, max_level, _ts_mode(fast_timesteps))
         h = C.c_void_p()
         check(_lib.hhs_server_create(C.byref(h), C.byref(cfg)))
         self.h = h

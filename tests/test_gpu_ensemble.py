@@ -113,3 +113,19 @@ def test_ensemble_tiny_systems_and_momentum(hhs, torch_cuda):
         m = systems[k][2]
         mom = (m[:, None] * v).sum(0)
         assert np.abs(mom).max() <= 1e-6 * (m[:, None] * np.abs(v)).sum(0).max()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_ensemble_timestep_modes_track_oracle(hhs, oracle, torch_cuda, mode):
+    eps, eta, t_end, dtr = 1e-2, 0.01, 1.0 / 16, 1.0 / 32
+    systems = _systems(hhs, [33, 300], 610, eps)
+    ens = hhs.Ensemble(systems, t_end, dtr, eta, eps, fast_timesteps=mode)
+    ens.run()
+    torch_cuda.cuda.synchronize()
+    res = ens.results()
+    for k, (p0, v0, m) in enumerate(systems):
+        host = oracle.HostParticles(len(m), p0, v0, m)
+        oracle.hold(host, t_end, dtr, eta, eps * eps)
+        assert res[k]["status"] == 0
+        p, _ = ens.state(k)
+        assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos ** 2).mean())

@@ -57,6 +57,25 @@ def test_hold_fast_timesteps_matches_oracle(hhs, oracle, torch_cuda):
     assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
 
 
+def test_hold_fp32_timesteps_tracks_oracle(hhs, oracle, torch_cuda):
+    """HOLD with FP32 timesteps: trajectories within 1e-6 of the rms and the
+    same energy error as the FP64 HOLD run to 10% (eta = 0.1)."""
+    n, eps, eta = 256, 1e-2, 0.1
+    pos, vel, m = plummer(hhs, n, seed=4343, eps2=eps * eps)
+    e0 = oracle.energy(pos, vel, m, eps * eps)
+    host = oracle.HostParticles(n, pos, vel, m)
+    rc, ost = oracle.hold(host, 0.25, 1.0 / 16, eta, eps * eps)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(n, eta, eps, fast_timesteps=hhs.TIMESTEP_FP32)
+    g.hold(d, 0.25, 1.0 / 16)
+    torch_cuda.cuda.synchronize()
+    p = d.pos.cpu().numpy()
+    assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
+    ke, pe = hhs.energy(d.pos, d.vel, d.m, eps * eps)
+    de_cpu = abs(oracle.energy(host.pos, host.vel, host.m, eps * eps) / e0 - 1)
+    assert abs(abs((ke + pe) / e0 - 1) - de_cpu) <= 0.1 * de_cpu
+
+
 def test_hold_energy_error_matches_fp64_run(hhs, oracle, torch_cuda):
     n, eps, eta = 256, 1e-2, 0.1
     pos, vel, m = plummer(hhs, n, seed=77, eps2=eps * eps)

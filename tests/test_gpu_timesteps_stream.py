@@ -68,6 +68,26 @@ def test_timesteps_fast_within_few_ulp(hhs, oracle, torch_cuda, n, eps2):
     assert np.max(np.abs(got / ref - 1.0)) <= 1e-14
 
 
+@pytest.mark.parametrize("n", [2, 65, 1024, 3000, 20000])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_timesteps_fp32_within_1e6(hhs, oracle, torch_cuda, n, eps2):
+    """FP32 mode (FP64 differences rounded once to FP32): relative
+    difference <= 1e-6 against the FP64 oracle (the timestep bar)."""
+    pos, vel, m = plummer(hhs, n, seed=190 + n, mass_model=1, eps2=eps2)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=0.01, eps2=eps2, fast=hhs.TIMESTEP_FP32)
+    torch_cuda.cuda.synchronize()
+    if n <= 3000:
+        ref = oracle.timesteps(pos, vel, m, 0.01, eps2)
+        got = d.timestep.cpu().numpy()
+    else:  # FP64 GPU exact mode is bit-exact with the oracle (tested above)
+        e = hhs.DeviceParticles.from_host(pos, vel, m)
+        hhs.timesteps(e, eta=0.01, eps2=eps2)
+        torch_cuda.cuda.synchronize()
+        ref, got = e.timestep.cpu().numpy(), d.timestep.cpu().numpy()
+    assert np.max(np.abs(got / ref - 1.0)) <= 1e-6
+
+
 def test_timesteps_fast_edge_cases(hhs, oracle, torch_cuda):
     # single particle: start value; cold start: fly-by terms never win;
     # coincident pair with eps = 0: tau = 0 exactly (as the IEEE form)
