@@ -131,10 +131,15 @@ int hhs_kick_set_config(int sinks_per_lane, int waves_per_group);
  *            (14 ops)
  * F64:       FP64 throughout with the hardware 1/sqrt estimate (relative
  *            error <= 2^-23) (12 ops)
- * All three meet the 1e-6 normalised force bar. */
+ * F64_EXACT: FP64 in the CPU direct sum's operation order with IEEE sqrt
+ *            and divide, each sink summing its sources in index order:
+ *            bit-exact with the FP64 CPU kick (src/kick.cuh:19-30 with the
+ *            attractive sign); one wave per group, several times slower.
+ * All four meet the 1e-6 normalised force bar. */
 #define HHS_KICK_MIXED32 0
 #define HHS_KICK_F64_RSQ32 1
 #define HHS_KICK_F64 2
+#define HHS_KICK_F64_EXACT 3
 int hhs_kick_set_precision(int mode);
 int hhs_kick_get_precision(void);
 
@@ -267,9 +272,15 @@ typedef struct hhs_hold_stats {
     long long calls_per_level[64];
 } hhs_hold_stats;
 int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out);
-/* HHS_TIMESTEP_EXACT (default) or HHS_TIMESTEP_FAST for hold_step's
- * timestep evaluations. */
+/* HHS_TIMESTEP_EXACT (default), HHS_TIMESTEP_FAST or HHS_TIMESTEP_FP32
+ * for hold_step's timestep evaluations. */
 int hhs_globals_set_timestep_mode(hhs_globals *g, int mode);
+/* Kick arithmetic for this hold_step (an HHS_KICK_* mode), or -1 (default)
+ * for the process-wide hhs_kick_set_precision setting.  HHS_KICK_F64_EXACT
+ * with HHS_TIMESTEP_EXACT is the FP64-only HOLD mode: every kernel of the
+ * recursion is bit-exact with the FP64 CPU HOLD, so the whole integration
+ * is.  No reference counterpart (the reference has one arithmetic per T). */
+int hhs_globals_set_kick_precision(hhs_globals *g, int mode);
 int hhs_globals_reset_stats(hhs_globals *g);
 
 /* hold_step<T,BLOCKSIZE> (src/hold.cuh:51-181) on total[0..n), at
@@ -317,7 +328,7 @@ typedef struct hhs_ensemble_desc {
     long long max_calls;   /* per-system cap on hold_step calls, 0 = none     */
     int wave_team_max_n;   /* 0 selects 256                                   */
     hhs_system_result *results; /* device, per system                        */
-    int timestep_mode;     /* HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST         */
+    int timestep_mode;     /* HHS_TIMESTEP_EXACT, _FAST or _FP32               */
 } hhs_ensemble_desc;
 
 size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles);
@@ -346,7 +357,7 @@ typedef struct hhs_server_config {
     int batch_streams;  /* ensemble batches in flight, 0 selects 2          */
     int big_workers;    /* concurrent large systems, 0 selects 2            */
     int max_level;      /* 0 selects 40                                     */
-    int timestep_mode;  /* HHS_TIMESTEP_EXACT or HHS_TIMESTEP_FAST          */
+    int timestep_mode;  /* HHS_TIMESTEP_EXACT, _FAST or _FP32                */
 } hhs_server_config;
 
 typedef struct hhs_request_info {

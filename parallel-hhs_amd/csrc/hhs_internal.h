@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "hhs.h"
 
@@ -70,6 +71,7 @@ template <typename T> struct KickArgs {
     int n1;
     T h;          // velocity update factor: vel += acc * h   (h = fac * dt)
     double eps2;
+    int precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
 };
 
 struct KickConfig {

@@ -10,7 +10,8 @@
 //    every call and never frees them (src/hold.cuh:56);
 //  * the S<-S and S<-F kicks of the slow particles are one launch over two
 //    source segments (slow, fast), so each slow particle's force is summed
-//    once (src/hold.cuh:137-145);
+//    once (src/hold.cuh:137-145); the exact FP64 kick mode keeps the two
+//    launches so its sums follow the CPU order;
 //  * the only host synchronisation per call is the 8-byte count read after
 //    the split (the reference blocks at the same point, src/split.cuh:430);
 //  * a max_level guard returns HHS_ERR_DEPTH instead of recursing without
@@ -44,6 +45,7 @@ struct hhs_globals {
     double simtime = 0.0;
     hhs_hold_stats stats{};
     int timestep_mode = HHS_TIMESTEP_EXACT;
+    int kick_precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
 };
 
 namespace hhs {
@@ -130,16 +132,26 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     if (ns > 0) {
         // src/hold.cuh:123-124
         if ((rc = launch_drift<double>(lb.slow.pos, lb.slow.vel, ns, dt * 0.5, st))) return rc;
-        // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow
+        // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow.
+        // In the exact FP64 mode the S<-S and S<-F kicks are two velocity
+        // updates, as in the reference, so the sums match the CPU order.
+        const bool exact = g->kick_precision == HHS_KICK_F64_EXACT;
         KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
                             lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
-                            lb.fast.m,   nf,         dt,          g->eps2};
+                            lb.fast.m,   exact ? 0 : nf, dt,      g->eps2,
+                            g->kick_precision};
         if ((rc = launch_kick<double>(ks, st))) return rc;
+        if (exact && nf > 0) {
+            KickArgs<double> ksf{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,      lb.fast.pos,
+                                 lb.fast.m,   nf,          nullptr,     nullptr, 0,
+                                 dt,          g->eps2,     g->kick_precision};
+            if ((rc = launch_kick<double>(ksf, st))) return rc;
+        }
         S.interactions += static_cast<long long>(ns) * (ns + nf);
         if (nf > 0) {
             KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
                                 lb.slow.m,   ns,          nullptr,     nullptr, 0,
-                                dt,          g->eps2};
+                                dt,          g->eps2,     g->kick_precision};
             if ((rc = launch_kick<double>(kf, st))) return rc;
             S.interactions += static_cast<long long>(nf) * ns;
         }
@@ -226,6 +238,14 @@ int hhs_globals_set_timestep_mode(hhs_globals *g, int mode) {
                     mode == HHS_TIMESTEP_FP32,
                 "set_timestep_mode: mode must be HHS_TIMESTEP_EXACT, _FAST or _FP32");
     g->timestep_mode = mode;
+    return HHS_OK;
+}
+
+int hhs_globals_set_kick_precision(hhs_globals *g, int mode) {
+    HHS_REQUIRE(g, "set_kick_precision: null globals");
+    HHS_REQUIRE(mode >= -1 && mode <= HHS_KICK_F64_EXACT,
+                "set_kick_precision: mode must be -1 or an HHS_KICK_* value");
+    g->kick_precision = mode;
     return HHS_OK;
 }
 

@@ -126,6 +126,34 @@ template <int S, int P, bool SOFT> struct KickAcc64 {
 template <int S, bool SOFT> struct KickAcc<double, S, 1, SOFT> : KickAcc64<S, 1, SOFT> {};
 template <int S, bool SOFT> struct KickAcc<double, S, 2, SOFT> : KickAcc64<S, 2, SOFT> {};
 
+//  P = 3: FP64 in the CPU direct sum's operation order with IEEE sqrt and
+//         divide; launched with one wave per group so each sink sums its
+//         sources in index order (bit-exact with the FP64 CPU sum).
+template <int S, bool SOFT> struct KickAcc<double, S, 3, SOFT> {
+    double ax[S], ay[S], az[S];
+    __device__ KickAcc() {
+#pragma unroll
+        for (int s = 0; s < S; ++s) ax[s] = ay[s] = az[s] = 0.0;
+    }
+    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm, int jb,
+                          int je, const double (&xi)[S], const double (&yi)[S],
+                          const double (&zi)[S], float, double eps2) {
+#pragma unroll 2
+        for (int j = jb; j < je; ++j) {
+            const double xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
+            const double mj = sm[j];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                pair_accel_exact(xj, yj, zj, mj, xi[s], yi[s], zi[s], eps2, ax[s], ay[s], az[s]);
+        }
+    }
+    __device__ void total(int s, double &x, double &y, double &z) const {
+        x = ax[s];
+        y = ay[s];
+        z = az[s];
+    }
+};
+
 // One workgroup: 64*S sinks (S per lane), W waves splitting the source list
 // (segment 0 then segment 1), per-wave totals merged through LDS in wave
 // order (deterministic), epilogue acc <- a, vel += a * h.
@@ -184,7 +212,10 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
             }
         }
     }
-    // Epilogue (src/kick.cuh:98-104): acc <- a, vel += a * (fac * dt).
+    // Epilogue (src/kick.cuh:98-104): acc <- a, vel += a * (fac * dt), as a
+    // multiply then an add (no contraction, like the FP64 CPU update).
+    {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int i = tile0 + s * 64 + lane;
@@ -200,6 +231,7 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
             si_vel[3 * i + 1] = si_vel[3 * i + 1] + a1 * h;
             si_vel[3 * i + 2] = si_vel[3 * i + 2] + a2 * h;
         }
+    }
     }
 }
 
@@ -226,6 +258,7 @@ int launch_sw(const KickArgs<T> &a, int P, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
         if (P == 1) return launch_swp<T, S, W, 1>(a, st);
         if (P == 2) return launch_swp<T, S, W, 2>(a, st);
+        if (P == 3) return launch_swp<T, S, W, 3>(a, st);
     }
     return launch_swp<T, S, W, 0>(a, st);
 }
@@ -284,8 +317,15 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     HHS_REQUIRE(a.si_pos && a.si_vel, "kick: null sink pos/vel");
     HHS_REQUIRE(a.n0 == 0 || (a.s0_pos && a.s0_m), "kick: null source segment 0");
     HHS_REQUIRE(a.n1 == 0 || (a.s1_pos && a.s1_m), "kick: null source segment 1");
-    const KickConfig c = kick_pick_config(a.n_sink, a.n0 + a.n1);
-    const int P = g_precision.load();
+    KickConfig c = kick_pick_config(a.n_sink, a.n0 + a.n1);
+    const int P = a.precision >= 0 ? a.precision : g_precision.load();
+    HHS_REQUIRE(P >= HHS_KICK_MIXED32 && P <= HHS_KICK_F64_EXACT, "kick: precision mode %d", P);
+    if (P == HHS_KICK_F64_EXACT && std::is_same<T, double>::value) {
+        // One lane per sink, one wave per group: each sink sums its sources
+        // in index order.  Segment 1 follows segment 0 in the same sum.
+        c.sinks_per_lane = 1;
+        c.waves = 1;
+    }
     switch (c.sinks_per_lane) {
     case 1: return launch_s<T, 1>(a, c.waves, P, st);
     case 2: return launch_s<T, 2>(a, c.waves, P, st);
@@ -360,8 +400,9 @@ int hhs_kick_set_config(int sinks_per_lane, int waves_per_group) {
 }
 
 int hhs_kick_set_precision(int mode) {
-    HHS_REQUIRE(mode == HHS_KICK_MIXED32 || mode == HHS_KICK_F64_RSQ32 || mode == HHS_KICK_F64,
-                "kick precision: mode must be HHS_KICK_MIXED32, HHS_KICK_F64_RSQ32 or HHS_KICK_F64");
+    HHS_REQUIRE(mode >= HHS_KICK_MIXED32 && mode <= HHS_KICK_F64_EXACT,
+                "kick precision: mode must be HHS_KICK_MIXED32, HHS_KICK_F64_RSQ32, HHS_KICK_F64 "
+                "or HHS_KICK_F64_EXACT");
     g_precision.store(mode);
     return HHS_OK;
 }

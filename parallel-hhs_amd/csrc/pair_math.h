@@ -156,6 +156,24 @@ __device__ __forceinline__ float separation(double xj, double xi) {
 }
 __device__ __forceinline__ float separation(float xj, float xi) { return xj - xi; }
 
+// One source's contribution in FP64, in the CPU direct sum's operation order
+// (src/kick.cuh:19-30 with the attractive sign of tests/test_nbforce.cu:14-18):
+// dx = x_j - x_i, d2 = dx*dx + dy*dy + dz*dz + eps2, skip d2 == 0,
+// s = m_j * (1 / (d2 * sqrt(d2))), a += dx * s.  IEEE sqrt and divide, no
+// contraction: summed in source order it is bit-exact with that sum.
+__device__ __forceinline__ void pair_accel_exact(double xj, double yj, double zj, double mj,
+                                                 double xi, double yi, double zi, double eps2,
+                                                 double &ax, double &ay, double &az) {
+#pragma clang fp contract(off)
+    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double s = mj * (1.0 / (d2 * sqrt(d2)));
+    const bool skip = d2 == 0.0;
+    ax = skip ? ax : ax + dx * s;
+    ay = skip ? ay : ay + dy * s;
+    az = skip ? az : az + dz * s;
+}
+
 // One source's contribution to a sink's acceleration (FP32 from FP64-derived
 // separations); SOFT = (eps2 > 0), else pairs with r^2 == 0 are skipped
 // (src/kick.cuh:25).

@@ -139,6 +139,7 @@ _SIGS = {
     "hhs_timesteps_fast_f64": (_i, [_P64, _i, _d, _d, _vp]),
     "hhs_timesteps_mode_f64": (_i, [_P64, _i, _d, _d, _i, _vp]),
     "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
+    "hhs_globals_set_kick_precision": (_i, [_vp, _i]),
     "hhs_scan_workspace_bytes": (_sz, [_i]),
     "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
     "hhs_scan_workspace_thread": (_i, [_i, C.POINTER(_vp), C.POINTER(_sz)]),
@@ -400,12 +401,17 @@ class Globals:
     """nbodysystem_globals<double> + per-level buffers (hhs_globals)."""
 
     def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48,
-                 fast_timesteps: bool = False):
+                 fast_timesteps: bool = False, kick_precision: int = -1):
+        """`kick_precision` is a KICK_* mode for this HOLD (-1: the process-wide
+        setting).  kick_precision=KICK_F64_EXACT with exact timesteps is the
+        FP64-only HOLD mode, bit-exact with the FP64 CPU HOLD."""
         h = C.c_void_p()
         check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
         self.h = h
         if _ts_mode(fast_timesteps) != TIMESTEP_EXACT:
             check(_lib.hhs_globals_set_timestep_mode(h, _ts_mode(fast_timesteps)))
+        if kick_precision != -1:
+            check(_lib.hhs_globals_set_kick_precision(h, kick_precision))
 
     def close(self):
         if self.h:
@@ -597,7 +603,7 @@ class Server:
                 "running_big": st.running_big}
 
 
-KICK_MIXED32, KICK_F64_RSQ32, KICK_F64 = 0, 1, 2
+KICK_MIXED32, KICK_F64_RSQ32, KICK_F64, KICK_F64_EXACT = 0, 1, 2, 3
 
 
 def kick_set_precision(mode: int):

@@ -65,6 +65,12 @@ def test_invalid_arguments_return_status(hhs):
     assert lib.hhs_hold_step_f64(None, None, 1, 0, 0.0, 1.0, 1.0, 1, None) == hhs.HHS_ERR_INVALID
     assert lib.hhs_ic_plummer(0, 1, 0, 0, 0.0, None, None, None) == hhs.HHS_ERR_INVALID
     assert "ic_plummer" in hhs.last_error()
+    assert lib.hhs_globals_set_kick_precision(None, 3) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_kick_set_precision(4) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_kick_set_precision(-1) == hhs.HHS_ERR_INVALID
+    assert lib.hhs_kick_set_precision(hhs.KICK_F64_EXACT) == hhs.HHS_OK
+    assert lib.hhs_kick_get_precision() == hhs.KICK_F64_EXACT
+    assert lib.hhs_kick_set_precision(hhs.KICK_MIXED32) == hhs.HHS_OK
     assert lib.hhs_scan_workspace_bytes(0) >= 64
     assert lib.hhs_scan_workspace_bytes(1 << 20) == 64 + 8 * (1 << 12)
 

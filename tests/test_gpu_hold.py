@@ -42,6 +42,28 @@ def test_hold_one_root_step_matches_oracle(hhs, oracle, torch_cuda, n):
     assert np.array_equal(d.m.cpu().numpy(), host.m)
 
 
+@pytest.mark.parametrize("n,eps", [(2, 0.0), (16, 1e-2), (129, 1e-2), (512, 1e-3), (1024, 0.0)])
+def test_hold_fp64_mode_bit_exact(hhs, oracle, torch_cuda, n, eps):
+    """FP64-only HOLD (exact timesteps + F64_EXACT kicks): the whole
+    integration -- recursion, positions, velocities, timesteps, simulated
+    time -- is bit-identical with the FP64 CPU HOLD."""
+    eta, t_end, dtr = 0.01, 1.0 / 16, 1.0 / 32
+    pos, vel, m = plummer(hhs, n, seed=2000 + n, eps2=eps * eps)
+    host = oracle.HostParticles(n, pos, vel, m)
+    rc, ost = oracle.hold(host, t_end, dtr, eta, eps * eps)
+    assert rc == 0
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(n, eta, eps, kick_precision=hhs.KICK_F64_EXACT)
+    g.hold(d, t_end, dtr)
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert st["calls"] == ost.calls and st["interactions"] == ost.interactions
+    assert g.simtime == ost.simtime
+    assert d.pos.cpu().numpy().tobytes() == host.pos.tobytes()
+    assert d.vel.cpu().numpy().tobytes() == host.vel.tobytes()
+    assert d.timestep.cpu().numpy().tobytes() == host.timestep.tobytes()
+
+
 def test_hold_fast_timesteps_matches_oracle(hhs, oracle, torch_cuda):
     """HOLD with FAST timesteps: same recursion and trajectories to 1e-6."""
     n, eps, eta = 256, 1e-2, 0.01

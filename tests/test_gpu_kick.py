@@ -79,6 +79,43 @@ def test_kick_precision_modes(hhs, oracle, torch_cuda, precision, eps2):
     assert hhs.kick_get_precision() == 0
 
 
+@pytest.mark.parametrize("n", [1, 2, 65, 1000, 3001])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_kick_f64_exact_bit_exact(hhs, oracle, torch_cuda, n, eps2):
+    """F64_EXACT: the oracle's operation order with IEEE sqrt/divide and the
+    sources summed in index order -- acceleration and velocity bit for bit,
+    including a coincident pair with eps2 = 0 (skipped, d2 == 0)."""
+    pos, vel, m = plummer(hhs, n, seed=350 + n, mass_model=1, eps2=eps2)
+    if n > 3:
+        pos[3] = pos[2]
+    d = _dev(hhs, pos, vel, m)
+    hhs.kick_set_precision(hhs.KICK_F64_EXACT)
+    try:
+        hhs.kick_self(d, dt=0.0078125, fac=0.5, eps2=eps2)
+        torch_cuda.cuda.synchronize()
+    finally:
+        hhs.kick_set_precision(0)
+    vel_ref, acc_ref = oracle.kick(pos, vel, pos, m, 0.0078125, 0.5, eps2)
+    assert d.acc.cpu().numpy().tobytes() == acc_ref.tobytes()
+    assert d.vel.cpu().numpy().tobytes() == vel_ref.tobytes()
+
+
+def test_kick_f64_exact_cross_bit_exact(hhs, oracle, torch_cuda):
+    eps2 = 1e-8
+    ps, vs, ms = plummer(hhs, 700, seed=21, eps2=eps2)
+    pk, vk, mk = plummer(hhs, 333, seed=22, eps2=eps2)
+    so, si = _dev(hhs, ps, vs, ms), _dev(hhs, pk, vk, mk)
+    hhs.kick_set_precision(hhs.KICK_F64_EXACT)
+    try:
+        hhs.kick_cross(so, si, dt=0.3, fac=1.0, eps2=eps2)
+        torch_cuda.cuda.synchronize()
+    finally:
+        hhs.kick_set_precision(0)
+    vel_ref, acc_ref = oracle.kick(pk, vk, ps, ms, 0.3, 1.0, eps2)
+    assert si.acc.cpu().numpy().tobytes() == acc_ref.tobytes()
+    assert si.vel.cpu().numpy().tobytes() == vel_ref.tobytes()
+
+
 @pytest.mark.parametrize("n_sink,n_src", [(1, 5), (300, 517), (517, 300), (64, 1), (129, 4096)])
 def test_kick_cross_matches_oracle(hhs, oracle, torch_cuda, n_sink, n_src):
     eps2 = 1e-8
