@@ -37,6 +37,8 @@ extern "C" {
 #define HHS_ERR_BUDGET 6    /* request exceeds the device-memory budget     */
 #define HHS_ERR_NONFINITE 7 /* NaN/Inf detected in a diagnostic             */
 #define HHS_ERR_WORKSPACE 8 /* workspace too small / not initialised        */
+#define HHS_ERR_IO 9        /* a snapshot file could not be opened/written  */
+#define HHS_ERR_FORMAT 10   /* not a snapshot, or a damaged one             */
 
 /* Message of the last failure on this thread ("" if none). */
 const char *hhs_last_error(void);
@@ -281,6 +283,12 @@ int hhs_globals_set_timestep_mode(hhs_globals *g, int mode);
  * recursion is bit-exact with the FP64 CPU HOLD, so the whole integration
  * is.  No reference counterpart (the reference has one arithmetic per T). */
 int hhs_globals_set_kick_precision(hhs_globals *g, int mode);
+int hhs_globals_timestep_mode(const hhs_globals *g);
+int hhs_globals_kick_precision(const hhs_globals *g);
+/* Root steps completed by hhs_evolve_f64 (restored from a snapshot on
+ * resume). */
+long long hhs_globals_root_steps(const hhs_globals *g);
+void hhs_globals_set_root_steps(hhs_globals *g, long long k);
 int hhs_globals_reset_stats(hhs_globals *g);
 
 /* hold_step<T,BLOCKSIZE> (src/hold.cuh:51-181) on total[0..n), at
@@ -296,6 +304,65 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
  * dt_root (timesteps recomputed at each root step) until t_end. */
 int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
                  void *stream);
+
+/* ------------------------------------------- driver and snapshots -- */
+/* Snapshot file (no reference counterpart: the reference exports state
+ * only through gpu_to_host, src/particle_system.cuh:230-247, which also
+ * leaves out parent_id).  Layout, little-endian:
+ *   hhs_snapshot_header   128 bytes
+ *   pos                   3n doubles (AoS x, y, z)
+ *   vel                   3n doubles
+ *   m                     n doubles
+ *   id                    n int32
+ * i.e. 128 + 60 n bytes.  Snapshots are taken only at root-step
+ * boundaries: there every particle is at simtime, and nothing else is
+ * carried from one root step to the next (timesteps are recomputed at the
+ * root, the per-level buffers are scratch), so an integration resumed from
+ * a snapshot continues bit for bit. */
+#define HHS_SNAPSHOT_VERSION 1
+typedef struct hhs_snapshot_header {
+    char magic[8];          /* "HHSSNAP" and a NUL                          */
+    uint32_t version;       /* HHS_SNAPSHOT_VERSION                         */
+    uint32_t header_bytes;  /* sizeof(hhs_snapshot_header) = 128            */
+    int64_t n;              /* particles                                    */
+    int64_t root_steps;     /* root steps completed                         */
+    double simtime;         /* globals simtime at the snapshot              */
+    double dt_root;         /* root step of the run that wrote it           */
+    double eta, eps;        /* integrator parameters                        */
+    int32_t timestep_mode;  /* HHS_TIMESTEP_*                               */
+    int32_t kick_precision; /* HHS_KICK_* or -1                             */
+    uint64_t payload_fnv1a; /* FNV-1a 64 of the payload bytes               */
+    uint8_t reserved[48];   /* zero                                         */
+} hhs_snapshot_header;
+
+/* Writes sys[0..n) (pos, vel, m; id if non-NULL, else 0..n-1) and g's
+ * state to `path` after the work queued on `stream` has finished.  Writes
+ * "<path>.tmp" and renames it over `path`, so an interrupted write leaves
+ * the previous snapshot intact.  Blocking. */
+int hhs_snapshot_write_f64(const char *path, const hhs_globals *g, const hhs_particles_f64 *sys,
+                           int n, double dt_root, void *stream);
+/* Reads and checks the header (magic, version, sizes against the file
+ * length); HHS_ERR_FORMAT when it does not describe a whole snapshot. */
+int hhs_snapshot_read_header(const char *path, hhs_snapshot_header *out);
+/* Loads a snapshot into device arrays sys (sys->n >= the snapshot's n;
+ * pos, vel, m required, id written when non-NULL), verifying the payload
+ * checksum.  If g is non-NULL its simtime and root-step count are set from
+ * the header, and its eta/eps and modes must equal the snapshot's
+ * (HHS_ERR_INVALID otherwise), so that the resumed run is the same
+ * integration.  `hdr` (optional) receives the header.  Blocking. */
+int hhs_snapshot_read_f64(const char *path, hhs_particles_f64 *sys, hhs_globals *g,
+                          hhs_snapshot_header *hdr, void *stream);
+
+/* Top-level driver: hold<T> (declared, never defined: src/hold.cuh:20-35)
+ * with root-step bookkeeping.  Runs root steps k = root_steps(g), ... of
+ * dt_root over [k dt_root, (k+1) dt_root] while k dt_root < t_end,
+ * recomputing the timesteps at each root step.  When snapshot_path is
+ * non-NULL, writes a snapshot after every snapshot_every-th root step
+ * (k+1 divisible by it; 0 = none on the way) and after the last one.
+ * Resume = hhs_snapshot_read_f64 into fresh arrays and globals, then
+ * hhs_evolve_f64 again with the same t_end and dt_root. */
+int hhs_evolve_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
+                   const char *snapshot_path, int snapshot_every, void *stream);
 
 /* --------------------------------------------------------- ensembles -- */
 /* Many independent systems integrated in one persistent launch (no

@@ -46,6 +46,7 @@ struct hhs_globals {
     hhs_hold_stats stats{};
     int timestep_mode = HHS_TIMESTEP_EXACT;
     int kick_precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
+    long long root_steps = 0; // root steps completed by hhs_evolve_f64
 };
 
 namespace hhs {
@@ -247,6 +248,13 @@ int hhs_globals_set_kick_precision(hhs_globals *g, int mode) {
                 "set_kick_precision: mode must be -1 or an HHS_KICK_* value");
     g->kick_precision = mode;
     return HHS_OK;
+}
+
+int hhs_globals_timestep_mode(const hhs_globals *g) { return g ? g->timestep_mode : -1; }
+int hhs_globals_kick_precision(const hhs_globals *g) { return g ? g->kick_precision : -1; }
+long long hhs_globals_root_steps(const hhs_globals *g) { return g ? g->root_steps : 0; }
+void hhs_globals_set_root_steps(hhs_globals *g, long long k) {
+    if (g) g->root_steps = k;
 }
 
 int hhs_globals_stats(const hhs_globals *g, hhs_hold_stats *out) {

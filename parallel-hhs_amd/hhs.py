@@ -28,6 +28,8 @@ HHS_ERR_DEPTH = 5
 HHS_ERR_BUDGET = 6
 HHS_ERR_NONFINITE = 7
 HHS_ERR_WORKSPACE = 8
+HHS_ERR_IO = 9
+HHS_ERR_FORMAT = 10
 
 
 class HHSError(RuntimeError):
@@ -99,6 +101,18 @@ class ServerStats(C.Structure):
                 ("pending", C.c_int), ("running_batches", C.c_int), ("running_big", C.c_int)]
 
 
+class SnapshotHeader(C.Structure):
+    """hhs_snapshot_header: the 128-byte header of a snapshot file."""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("header_bytes", C.c_uint32),
+                ("n", C.c_int64), ("root_steps", C.c_int64), ("simtime", C.c_double),
+                ("dt_root", C.c_double), ("eta", C.c_double), ("eps", C.c_double),
+                ("timestep_mode", C.c_int32), ("kick_precision", C.c_int32),
+                ("payload_fnv1a", C.c_uint64), ("reserved", C.c_uint8 * 48)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f not in ("magic", "reserved")}
+
+
 class HoldStats(C.Structure):
     _fields_ = [("calls", C.c_longlong), ("interactions", C.c_longlong),
                 ("ts_pairs", C.c_longlong), ("max_level", C.c_int),
@@ -140,6 +154,14 @@ _SIGS = {
     "hhs_timesteps_mode_f64": (_i, [_P64, _i, _d, _d, _i, _vp]),
     "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
     "hhs_globals_set_kick_precision": (_i, [_vp, _i]),
+    "hhs_globals_timestep_mode": (_i, [_vp]),
+    "hhs_globals_kick_precision": (_i, [_vp]),
+    "hhs_globals_root_steps": (C.c_longlong, [_vp]),
+    "hhs_globals_set_root_steps": (None, [_vp, C.c_longlong]),
+    "hhs_snapshot_write_f64": (_i, [C.c_char_p, _vp, _P64, _i, _d, _vp]),
+    "hhs_snapshot_read_header": (_i, [C.c_char_p, C.POINTER(SnapshotHeader)]),
+    "hhs_snapshot_read_f64": (_i, [C.c_char_p, _P64, _vp, C.POINTER(SnapshotHeader), _vp]),
+    "hhs_evolve_f64": (_i, [_vp, _P64, _d, _d, C.c_char_p, _i, _vp]),
     "hhs_scan_workspace_bytes": (_sz, [_i]),
     "hhs_scan_workspace_init": (_i, [_vp, _sz, _vp]),
     "hhs_scan_workspace_thread": (_i, [_i, C.POINTER(_vp), C.POINTER(_sz)]),
@@ -452,6 +474,139 @@ class Globals:
     def hold(self, sys: DeviceParticles, t_end: float, dt_root: float, stream=None):
         s = sys.struct()
         check(_lib.hhs_hold_f64(self.h, C.byref(s), t_end, dt_root, _stream(stream)))
+
+    @property
+    def root_steps(self) -> int:
+        return _lib.hhs_globals_root_steps(self.h)
+
+    def evolve(self, sys: DeviceParticles, t_end: float, dt_root: float,
+               snapshot_path: str | None = None, snapshot_every: int = 0, stream=None):
+        """hhs_evolve_f64: root steps until t_end, snapshots at root-step
+        boundaries (every `snapshot_every` steps and after the last)."""
+        s = sys.struct()
+        path = None if snapshot_path is None else os.fsencode(snapshot_path)
+        check(_lib.hhs_evolve_f64(self.h, C.byref(s), t_end, dt_root, path, snapshot_every,
+                                  _stream(stream)))
+
+    def save_snapshot(self, path: str, sys: DeviceParticles, dt_root: float, n=None,
+                      stream=None):
+        s = sys.struct()
+        check(_lib.hhs_snapshot_write_f64(os.fsencode(path), self.h, C.byref(s),
+                                          sys.n if n is None else n, dt_root, _stream(stream)))
+
+    def load_snapshot(self, path: str, sys: DeviceParticles, stream=None) -> dict:
+        """Loads particles into `sys` and this globals' simtime and root-step
+        count (eta, eps and modes must match the snapshot's)."""
+        s = sys.struct()
+        h = SnapshotHeader()
+        check(_lib.hhs_snapshot_read_f64(os.fsencode(path), C.byref(s), self.h, C.byref(h),
+                                         _stream(stream)))
+        return h.as_dict()
+
+
+def read_snapshot_header(path: str) -> dict:
+    h = SnapshotHeader()
+    check(_lib.hhs_snapshot_read_header(os.fsencode(path), C.byref(h)))
+    return h.as_dict()
+
+
+class HoldIntegrator:
+    """AMUSE-style front end: numpy arrays in and out, `evolve_model(t)` in
+    root steps of dt_root, snapshots and resume.  Device memory comes from
+    the C ABI (hhs_malloc); PyTorch is not used.
+
+    timestep_mode: TIMESTEP_EXACT / _FAST / _FP32; kick_precision: a KICK_*
+    mode or -1 (process-wide setting).  KICK_F64_EXACT with TIMESTEP_EXACT
+    integrates bit-identically with the FP64 CPU HOLD."""
+
+    _FIELDS = (("pos", 24), ("vel", 24), ("acc", 24), ("timestep", 8), ("m", 8), ("id", 4),
+               ("parent_id", 4))
+
+    def __init__(self, pos, vel, m, eta=0.01, eps=0.0, dt_root=1.0 / 64, max_level=48,
+                 timestep_mode=TIMESTEP_EXACT, kick_precision=-1, ids=None, _load=None):
+        import numpy as np
+        self.dt_root = float(dt_root)
+        self.eps = float(eps)
+        self._ptrs = {}
+        if _load is not None:
+            hdr = read_snapshot_header(_load)
+            n = int(hdr["n"])
+            eta, self.eps, self.dt_root = hdr["eta"], hdr["eps"], hdr["dt_root"]
+            timestep_mode, kick_precision = hdr["timestep_mode"], hdr["kick_precision"]
+        else:
+            n = len(m)
+        self.n = n
+        try:
+            for f, b in self._FIELDS:
+                p = C.c_void_p()
+                check(_lib.hhs_malloc(C.byref(p), max(b * n, 1)))
+                self._ptrs[f] = p
+            self.globals = Globals(n, eta, self.eps, max_level, timestep_mode, kick_precision)
+        except Exception:
+            self.close()
+            raise
+        if _load is not None:
+            self.globals.load_snapshot(_load, self, stream=0)
+            return
+        ids = np.arange(n, dtype=np.int32) if ids is None else ids
+        for f, a, dt in (("pos", pos, np.float64), ("vel", vel, np.float64),
+                         ("m", m, np.float64), ("id", ids, np.int32)):
+            a = np.ascontiguousarray(a, dtype=dt)
+            check(_lib.hhs_memcpy_h2d(self._ptrs[f], a.ctypes.data, a.nbytes))
+
+    @classmethod
+    def load(cls, path: str, max_level: int = 48) -> "HoldIntegrator":
+        """Resume from a snapshot: same n, eta, eps, modes, dt_root and
+        model time as the run that wrote it."""
+        return cls(None, None, None, max_level=max_level, _load=path)
+
+    def struct(self) -> ParticlesF64:
+        p = self._ptrs
+        return ParticlesF64(p["pos"], p["vel"], p["acc"], p["timestep"], p["m"], p["id"],
+                            p["parent_id"], self.n)
+
+    @property
+    def model_time(self) -> float:
+        return self.globals.root_steps * self.dt_root
+
+    def evolve_model(self, t_end: float, snapshot_path: str | None = None,
+                     snapshot_every: int = 0):
+        self.globals.evolve(self, t_end, self.dt_root, snapshot_path, snapshot_every, stream=0)
+
+    def save(self, path: str):
+        self.globals.save_snapshot(path, self, self.dt_root, stream=0)
+
+    def get_state(self):
+        """(pos[n,3], vel[n,3], m[n], id[n]) copied to the host."""
+        import numpy as np
+        out = (np.empty((self.n, 3)), np.empty((self.n, 3)), np.empty(self.n),
+               np.empty(self.n, dtype=np.int32))
+        check(_lib.hhs_stream_synchronize(None))
+        for f, a in zip(("pos", "vel", "m", "id"), out):
+            check(_lib.hhs_memcpy_d2h(a.ctypes.data, self._ptrs[f], a.nbytes))
+        return out
+
+    def energy(self):
+        """(kinetic, potential) in FP64 on the device."""
+        ke, pe = C.c_double(), C.c_double()
+        p = self._ptrs
+        check(_lib.hhs_energy_f64(p["pos"], p["vel"], p["m"], self.n, self.eps * self.eps,
+                                  C.byref(ke), C.byref(pe), None))
+        return ke.value, pe.value
+
+    def close(self):
+        g = getattr(self, "globals", None)
+        if g is not None:
+            g.close()
+        for p in getattr(self, "_ptrs", {}).values():
+            _lib.hhs_free(p)
+        self._ptrs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ic_plummer(n: int, seed: int, mass_model: int = 0, virialize: bool = True,
