@@ -7,13 +7,19 @@ N^2 = 1.718e10 interactions, acc written and vel += acc * dt.  `value` is
 Ginteractions/s over the timed steps (inputs resident in HBM).  The roofline
 object prices the kick at 20 flops per interaction against the FP32 vector
 peak.  Beside the headline line the JSON carries:
-  * hold_root_step -- one HOLD root step (dt = 1/64) of the same system;
+  * timesteps -- the timestep kernel over the same system, three modes;
+  * stream_kernels -- split, scan, drift and update against HBM bandwidth
+    at N = 131072 and 2^20..2^24;
+  * hold_root_step -- one HOLD root step (dt = 1/64) of the same system,
+    beside the FP64-only HOLD of the same step (the energy reference);
   * ensemble -- 4096 Plummer systems (N log-uniform in [8, 2048], t = 1) in
     one persistent launch, against the plain one-system-at-a-time loop;
   * served -- a Poisson request stream through the continuous-batching
     server, p50/p99 time-to-solution;
   * cpu_baseline -- the FP64 oracle direct sum on the host, one thread, on
-    a bounded sample of sinks.
+    a bounded sample of sinks, and the kick's force error on that sample;
+  * cpu_reference -- all-threads CPU kick, the reference's cpu_split, and
+    the FP64 CPU HOLD at N = 1024 beside the GPU.
 """
 import argparse
 import json
@@ -32,7 +38,10 @@ EPS = 1e-4
 ETA = 0.01
 SEED = 3
 FLOPS_PER_INTERACTION = 20
+TS_FLOPS_PER_PAIR = 49
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
 
 
 def log(msg):
@@ -70,23 +79,112 @@ def cpu_baseline(pos, m, acc_gpu, eps2, seconds=12.0):
     t0 = time.perf_counter()
     ref = oracle.accel_subset(pos, idx, pos, m, eps2)
     dt = time.perf_counter() - t0
-    # precision of the benchmarked kick on the same sample
-    scale = np.empty(256)
-    for q, i in enumerate(idx[:256]):
+    # precision of the benchmarked kick on the same sample: error relative to
+    # sum_j m_j / (r_ij^2 + eps^2) (normalised) and to |a_i| (plain)
+    q = min(k, 1024)
+    scale = np.empty(q)
+    for j, i in enumerate(idx[:q]):
         d = pos - pos[i]
         r2 = (d * d).sum(1) + eps2
-        scale[q] = (m / r2).sum()
-    err = np.linalg.norm(acc_gpu[idx[:256]] - ref[:256], axis=1) / scale
+        scale[j] = (m / r2).sum()
+    diff = np.linalg.norm(acc_gpu[idx[:q]] - ref[:q], axis=1)
+    err = diff / scale
+    rel = diff / np.linalg.norm(ref[:q], axis=1)
+    ferr = {"sinks": q, "normalised_max": float(err.max()),
+            "normalised_rms": float(np.sqrt((err ** 2).mean())),
+            "relative_max": float(rel.max()), "relative_rms": float(np.sqrt((rel ** 2).mean()))}
     return {"value": round(k * N / dt / 1e9, 4), "unit": "Ginteractions/s", "cores": 1,
             "kind": "port",
             "sample": f"{k} random sinks x {N} sources, FP64 oracle direct sum "
                       "(oracle/hhs_oracle.c), one host thread",
-            "seconds": round(dt, 3)}, float(err.max())
+            "seconds": round(dt, 3)}, ferr
 
 
-def hold_bench(hhs, torch, pos, vel, m, dt):
+def box_info():
+    """Host facts of the machine the CPU legs ran on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "cpu_model": model}
+
+
+def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
+    """Further CPU timings (steady clock): the oracle direct sum with every
+    host thread; the reference's own cpu_split (src/cpu_scan.h:20-50,
+    compiled from the reference sources into oracle/_ref when present, else
+    the oracle's restatement), median of 5 on 2^24 doubles; and the FP64 CPU
+    HOLD at N = 1024 (eps = 0, eta = 0.01, t = 1 in root steps of 1/16)
+    beside the GPU FP64-only and default HOLD on the same system."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {"box": box_info()}
+    threads = out["box"]["affinity_cpus"]
+    oracle.set_threads(threads)
+    rng = np.random.default_rng(1)
+    probe = rng.choice(N, 256, replace=False)
+    t0 = time.perf_counter()
+    oracle.accel_subset(pos, probe, pos, m, eps2)
+    dtp = time.perf_counter() - t0
+    k = int(min(N, max(256, 256 * kick_seconds / max(dtp, 1e-6))))
+    idx = rng.choice(N, k, replace=False)
+    t0 = time.perf_counter()
+    oracle.accel_subset(pos, idx, pos, m, eps2)
+    dt = time.perf_counter() - t0
+    out["kick_all_threads"] = {"value": round(k * N / dt / 1e9, 4), "unit": "Ginteractions/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{k} random sinks x {N} sources"}
+    vals = np.random.default_rng(2).random(1 << 24)
+    times = []
+    use_ref = oracle.ref_available()
+    for _ in range(5):
+        t0 = time.perf_counter()
+        if use_ref:
+            oracle.ref_cpu_split(vals, 0.5)
+        else:
+            oracle.cpu_split(vals, 0.5)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    out["cpu_split"] = {"n": 1 << 24, "median_ms": round(med * 1e3, 2),
+                        "Melements_per_s": round((1 << 24) / med / 1e6, 1), "cores": 1,
+                        "kind": "reference" if use_ref else "port",
+                        "note": "timing includes one copy of the input array"}
+    oracle.set_threads(1)
+    p1, v1, m1 = hhs.ic_plummer(1024, 1, 0, True, 0.0)
+    host = oracle.HostParticles(1024, p1, v1, m1)
+    t0 = time.perf_counter()
+    rc, ost = oracle.hold(host, 1.0, 1.0 / 16, ETA, 0.0)
+    t_cpu = time.perf_counter() - t0
+    gpu = {}
+    for label, kw in (("fp64", {"kick_precision": hhs.KICK_F64_EXACT}), ("default", {})):
+        d1 = hhs.DeviceParticles.from_host(p1, v1, m1)
+        g1 = hhs.Globals(1024, ETA, 0.0, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g1.hold(d1, 1.0, 1.0 / 16)
+        torch.cuda.synchronize()
+        gpu[label] = {"seconds": round(time.perf_counter() - t0, 3), "calls": g1.stats()["calls"]}
+        if label == "fp64":
+            gpu[label]["bit_identical_to_cpu"] = bool(
+                d1.pos.cpu().numpy().tobytes() == host.pos.tobytes()
+                and d1.vel.cpu().numpy().tobytes() == host.vel.tobytes())
+        g1.close()
+    out["hold_n1024"] = {"cpu_seconds": round(t_cpu, 3), "cpu_threads": 1, "rc": rc,
+                         "calls": ost.calls, "levels": ost.max_level + 1,
+                         "interactions": ost.interactions, "gpu": gpu}
+    return out
+
+
+def hold_bench(hhs, torch, pos, vel, m, dt, label, **modes):
+    """One HOLD root step of the N = 131072 system; returns the summary and
+    the final positions (for the comparison with the FP64-only run)."""
     d2 = hhs.DeviceParticles.from_host(pos, vel, m)
-    g = hhs.Globals(N, ETA, EPS, fast_timesteps=True)
+    g = hhs.Globals(N, ETA, EPS, **modes)
     ke0, pe0 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
     torch.cuda.synchronize()
     h0 = time.perf_counter()
@@ -96,22 +194,110 @@ def hold_bench(hhs, torch, pos, vel, m, dt):
     ke1, pe1 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
     st = g.stats()
     g.close()
-    return {"dt": dt, "timesteps": "fast", "seconds": round(hs, 4), "calls": st["calls"],
+    return {"dt": dt, "mode": label, "seconds": round(hs, 4), "calls": st["calls"],
             "levels": st["max_level"] + 1, "interactions": st["interactions"],
             "ts_pairs": st["ts_pairs"],
-            "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0))}
+            "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0))}, d2.pos.cpu().numpy()
+
+
+def timesteps_bench(hhs, torch, d, stream, reps=3):
+    """The timestep kernel over all N = 131072 particles in its three modes,
+    priced at 49 flops per ordered pair (the reference formula counting
+    each +, -, *, /, sqrt and min as one)."""
+    out = []
+    pairs = float(N) * (N - 1)
+    for name, mode, peak in (("exact", 0, FP64_PEAK_TFLOPS), ("fast", 1, FP64_PEAK_TFLOPS),
+                             ("fp32", 2, FP32_PEAK_TFLOPS)):
+        hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=mode)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=mode)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        tf = TS_FLOPS_PER_PAIR * pairs / (ms * 1e-3) / 1e12
+        out.append({"mode": name, "ms": round(ms, 3),
+                    "Gpairs_per_s": round(pairs / (ms * 1e-3) / 1e9, 1),
+                    "TFLOP_per_s": round(tf, 2), "peak": peak, "frac": round(tf / peak, 4)})
+    return {"n": N, "pairs": "ordered, i != j", "flops_per_pair": TS_FLOPS_PER_PAIR,
+            "modes": out}
+
+
+def stream_bench(hhs, torch, stream, sizes=(N, 1 << 20, 1 << 22, 1 << 24), reps=10):
+    """HBM-bound kernels at the HOLD size and a large-N sweep, priced at
+    their algorithmic bytes: fused split 148 B/particle (8 B timestep for
+    the predicate + 68 B record read + 72 B record and parent_id written),
+    stand-alone scan 8 B/element, drift 72 B/particle, update 132 B/particle
+    (68 B read + 64 B scattered write; timed both as the scatter-back of a
+    split's halves, the HOLD pattern, and under a random permutation)."""
+    rows = []
+
+    def timed(fn):
+        fn()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    for n in sizes:
+        gen = torch.Generator(device="cuda").manual_seed(n)
+
+        def rand_set():
+            p = hhs.DeviceParticles.empty(n)
+            for f in ("pos", "vel", "m", "timestep"):
+                t = getattr(p, f)
+                t.copy_(torch.rand(t.shape, generator=gen, dtype=torch.float64, device="cuda"))
+            p.id.copy_(torch.arange(n, dtype=torch.int32, device="cuda"))
+            return p
+
+        tot, cur = rand_set(), rand_set()
+        cur.parent_id.copy_(torch.randperm(n, generator=gen, device="cuda").to(torch.int32))
+        slow, fast = hhs.DeviceParticles.empty(n), hhs.DeviceParticles.empty(n)
+        prev = hhs.DeviceParticles.empty(n)
+        ws = hhs.ScanWorkspace(n)
+        part = torch.zeros(2, dtype=torch.int32, device="cuda")
+        pred = torch.randint(0, 2, (n,), generator=gen, dtype=torch.int32, device="cuda")
+        scanned = torch.empty_like(pred)
+        row = {"n": n}
+        hhs.split(tot, slow, fast, 0.5, part, ws)
+        torch.cuda.synchronize()
+        ns, nf = (int(x) for x in part.cpu())
+
+        def update_split():  # the HOLD pattern: scatter back both halves of a split
+            hhs.update(prev, slow, n=ns)
+            hhs.update(prev, fast, n=nf)
+
+        for name, fn, bpp in (
+                ("split", lambda: hhs.split(tot, slow, fast, 0.5, part, ws), 148),
+                ("scan", lambda: hhs.scan_exclusive(pred, scanned, ws), 8),
+                ("drift", lambda: hhs.drift(tot.pos, tot.vel, dt=1e-9, fac=0.5), 72),
+                ("update", update_split, 132),
+                ("update_random_perm", lambda: hhs.update(prev, cur), 132)):
+            ms = timed(fn)
+            gbs = bpp * n / (ms * 1e-3) / 1e9
+            row[name] = {"us": round(ms * 1e3, 2), "GB_per_s": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"stream kernels n={n}: {row}")
+        rows.append(row)
+        del tot, cur, slow, fast, prev, ws, pred, scanned
+    return {"bytes_per_particle": {"split": 148, "scan": 8, "drift": 72, "update": 132},
+            "peak_GB_per_s": HBM_PEAK_GBS, "sizes": rows}
 
 
 def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
-    """Configuration 4: 4096 Plummer systems, N log-uniform in [8, 2048],
-    equal masses, eta = 0.01, t = 1, eps = 1e-4 (unspecified there; stated
-    here), root steps of 1/16, FAST timesteps, one persistent launch."""
+    """Ensemble workload: 4096 Plummer systems, N log-uniform in [8, 2048],
+    equal masses, eta = 0.01, t = 1, eps = 1e-4, root steps of 1/16, FP32
+    timesteps, one persistent launch."""
     rng = np.random.default_rng(4)
     sizes = np.exp(rng.uniform(np.log(8), np.log(2048), n_systems)).astype(int)
     systems = [hhs.ic_plummer(int(n), 4000 + k, 0, True, EPS * EPS)
                for k, n in enumerate(sizes)]
     t_end, dtr = 1.0, 1.0 / 16
-    ens = hhs.Ensemble(systems, t_end, dtr, ETA, EPS, fast_timesteps=True)
+    ens = hhs.Ensemble(systems, t_end, dtr, ETA, EPS, fast_timesteps=hhs.TIMESTEP_FP32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ens.run()
@@ -127,7 +313,7 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
     for k in pick:
         p, v, mm = systems[k]
         dd = hhs.DeviceParticles.from_host(p, v, mm)
-        g = hhs.Globals(len(mm), ETA, EPS, fast_timesteps=True)
+        g = hhs.Globals(len(mm), ETA, EPS, fast_timesteps=hhs.TIMESTEP_FP32)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g.hold(dd, t_end, dtr)
@@ -137,7 +323,7 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
     loop_per_system = float(np.mean(loop_t)) if loop_t else float("nan")
     sps = n_systems / wall
     return {"systems": n_systems, "n_range": [8, 2048], "t_end": t_end, "dt_root": dtr,
-            "eta": ETA, "eps": EPS, "timesteps": "fast", "seconds": round(wall, 3),
+            "eta": ETA, "eps": EPS, "timesteps": "fp32", "seconds": round(wall, 3),
             "systems_per_s": round(sps, 2),
             "systems_per_s_at_dE_le_1e-6": round(float(np.sum(ok & (de <= 1e-6))) / wall, 2),
             "frac_dE_le_1e-6": round(float(np.mean(de <= 1e-6)), 4),
@@ -153,7 +339,7 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
 
 
 def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
-    """Configuration 5 shape: Poisson arrivals of Plummer integrations with N
+    """Served workload: Poisson arrivals of Plummer integrations with N
     log-uniform in [n_lo, n_hi], end time uniform in [0.1, 1], eta = 0.01,
     through the continuous-batching server; time-to-solution = completion -
     arrival (server clock)."""
@@ -167,7 +353,7 @@ def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
     for rate in rates:
         gaps = np.random.default_rng(seed + int(rate * 1000)).exponential(1.0 / rate, n_req)
         with hhs.Server(ETA, EPS, hbm_budget_bytes=64 << 30, small_max_n=2048, max_n=n_hi,
-                        batch_streams=2, big_workers=2, fast_timesteps=True) as srv:
+                        batch_streams=2, big_workers=2, fast_timesteps=hhs.TIMESTEP_FP32) as srv:
             tickets = []
             t0 = time.perf_counter()
             t_next = t0
@@ -191,7 +377,7 @@ def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
                     "wall_s": round(wall, 3), "peak_admitted_bytes": int(peak),
                     "frac_dE_le_1e-6": round(float(np.mean(de <= 1e-6)), 4)})
     return {"n_range": [n_lo, n_hi], "t_end_range": [0.1, 1.0], "dt_root": 1.0 / 16,
-            "eta": ETA, "eps": EPS, "timesteps": "fast", "sweep": out}
+            "eta": ETA, "eps": EPS, "timesteps": "fp32", "sweep": out}
 
 
 def main():
@@ -204,6 +390,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in kick heuristic")
+    ap.add_argument("--no-stream", action="store_true", help="skip the split/scan/drift/update sweep")
+    ap.add_argument("--no-fp64-hold", action="store_true",
+                    help="skip the FP64-only HOLD root step (the energy reference)")
+    ap.add_argument("--no-cpu-extra", action="store_true",
+                    help="skip the all-threads, cpu_split and N=1024 CPU HOLD timings")
     ap.add_argument("--loop-sample", type=int, default=12,
                     help="systems timed through the plain per-system loop (0 skips it)")
     ap.add_argument("--serve-full", action="store_true",
@@ -213,8 +404,8 @@ def main():
                          "HBM traffic for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; the kick shards by sinks "
-              "(see DESIGN.md) but no multi-GPU path is built", file=sys.stderr)
+        print("bench.py: measures one GPU only; the kick would shard by sinks "
+              "across GPUs but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -291,9 +482,23 @@ def main():
         "ic_seconds": round(t_ic, 2),
     }
     acc_gpu = d.acc.cpu().numpy()
+    out["timesteps"] = timesteps_bench(hhs, torch, d, stream)
+    log(f"timesteps: {out['timesteps']}")
+    del d
+    if not args.no_stream:
+        out["stream_kernels"] = stream_bench(hhs, torch, stream)
     if not args.no_hold:
-        out["hold_root_step"] = hold_bench(hhs, torch, pos, vel, m, dt)
+        out["hold_root_step"], p_mixed = hold_bench(
+            hhs, torch, pos, vel, m, dt, "mixed kick, FP32 timesteps",
+            fast_timesteps=hhs.TIMESTEP_FP32)
         log(f"hold: {out['hold_root_step']}")
+        if not args.no_fp64_hold:
+            ref, p64 = hold_bench(hhs, torch, pos, vel, m, dt, "FP64-only (exact kick and "
+                                  "timesteps)", kick_precision=hhs.KICK_F64_EXACT)
+            ref["max_pos_diff_over_rms"] = float(np.abs(p_mixed - p64).max()
+                                                 / np.sqrt((p64 ** 2).mean()))
+            out["hold_root_step"]["fp64_reference"] = ref
+            log(f"hold fp64: {ref}")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample)
         log(f"ensemble: {out['ensemble']}")
@@ -308,7 +513,11 @@ def main():
     if not args.no_cpu:
         cb, ferr = cpu_baseline(pos, m, acc_gpu, eps2)
         out["cpu_baseline"] = cb
-        out["force_err_max_sampled"] = ferr
+        out["force_error"] = ferr
+        log(f"cpu baseline: {cb}")
+        if not args.no_cpu_extra:
+            out["cpu_reference"] = cpu_reference(hhs, torch, pos, m, eps2)
+            log(f"cpu reference: {out['cpu_reference']}")
     print(json.dumps(out), flush=True)
 
 

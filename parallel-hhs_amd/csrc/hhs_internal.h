@@ -90,8 +90,8 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
                           double *ts, int n, double eta, double eps2, hipStream_t s);
 
 // ----------------------------------------------------------- scan / split
-// Workspace layout: u32 header[4] = {tile counter, epoch, raw fast count, 0}
-// followed by one u64 status word per tile.
+// Workspace layout: u32 header[16] (header[2] = raw fast count of the last
+// split) followed by one int count per chunk of the reduce-then-scan.
 constexpr size_t kScanHeaderBytes = 64;
 size_t scan_ws_bytes(int n_max);
 int scan_ws_init(void *ws, size_t bytes, hipStream_t s);

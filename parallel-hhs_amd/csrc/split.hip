@@ -1,71 +1,48 @@
-// Single-pass exclusive scan and fused slow/fast split for gfx950.
+// Exclusive scan and fused slow/fast split for gfx950.
 //
 // Reference: _get_predicate, the Blelloch scans, add and _set_predicate
 // (src/split.cuh:107-610).  The reference runs 3-7 launches, 2 cudaMalloc
 // and 2 cudaFree per level plus a blocking cudaDeviceSynchronize
-// (src/split.cuh:430, :513-514, :563-564).  Here one kernel does
-// predicate + scan + stable scatter + counts, using decoupled look-back
-// (Merrill & Garland 2016, "Single-pass Parallel Prefix Scan with
-// Decoupled Look-back"):
+// (src/split.cuh:430, :513-514, :563-564).  Here the predicate, the scan,
+// the stable scatter and the counts are one reduce-then-scan over large
+// contiguous chunks:
 //
-//  * a workgroup takes the next tile index from an atomic counter, so tiles
-//    are numbered in the order they start and a tile only ever waits on
-//    tiles that are already running (forward progress);
-//  * the tile is loaded striped (element base + k*256 + t for item k), so
-//    consecutive lanes touch consecutive elements (coalesced HBM access),
-//    and scanned in index order with wave ballots / shuffles plus LDS;
-//  * the tile publishes its aggregate, walks back over predecessors'
-//    64-bit status words (epoch | flag | value, agent-scope acquire loads),
-//    then publishes its inclusive prefix;
-//  * status words carry a 30-bit epoch so the workspace needs no clearing
-//    between calls; the last tile resets the counter and bumps the epoch.
+//  * count_kernel: workgroup g sums the values (predicates) of chunk g and
+//    writes counts[g];
+//  * scan_chunks_kernel: workgroup g adds up counts[0..g) for its offset
+//    (at most kMaxChunks ints, read from L2), then walks its chunk in tiles
+//    of kBlock * kItems elements in index order -- striped loads (element
+//    base + k*kBlock + t), wave ballots / shuffles plus LDS for the tile
+//    scan -- and emits every element with its exclusive prefix; the last
+//    chunk reports the grand total.
+//
+// No inter-workgroup waiting, no device-scope atomics and no state kept
+// between calls: on MI355X a device-scope round trip crosses XCDs
+// (hundreds of ns), which made a chained (decoupled look-back) scan
+// latency-bound.  The cost is one extra read of the predicate source
+// (8 B per particle for the split).  Inputs of at most two tiles take one
+// workgroup and one launch.
 //
 // The lone-fast-particle rule (src/split.cuh:321-324) needs the final count,
-// which only the last tile knows; a follow-up kernel rewrites the slow set
+// which only the last chunk knows; a follow-up kernel rewrites the slow set
 // as a plain copy when the raw fast count is 1 (it exits at once otherwise).
 #include "hhs_internal.h"
 
 namespace hhs {
 namespace {
 
-constexpr int kBlock = 256;  // threads per tile
+constexpr int kBlock = 256;  // threads per workgroup
 constexpr int kWaves = kBlock / 64;
+constexpr int kItems = 4;    // elements per lane per tile
+constexpr int kTile = kBlock * kItems;
+constexpr int kMaxChunks = 2048;
 
-constexpr uint32_t kFlagAggregate = 1u;
-constexpr uint32_t kFlagPrefix = 2u;
-constexpr uint32_t kEpochMask = 0x3FFFFFFFu;
-
+// Workspace: u32 header[16] ([2] = raw fast count of the last split), then
+// one int count per chunk.
 struct ScanWs {
-    uint32_t *header;  // [0] tile counter, [1] epoch, [2] raw fast count
-    uint64_t *status;  // one word per tile
+    uint32_t *header;
+    int *counts;
 };
-
-__device__ __forceinline__ uint64_t make_status(uint32_t epoch, uint32_t flag, uint32_t v) {
-    return (static_cast<uint64_t>(epoch & kEpochMask) << 34) |
-           (static_cast<uint64_t>(flag) << 32) | static_cast<uint64_t>(v);
-}
-
-__device__ __forceinline__ void publish(uint64_t *p, uint64_t w) {
-    __hip_atomic_store(p, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Exclusive prefix of the tiles before `tile` (single lane).
-__device__ int look_back(const uint64_t *status, int tile, uint32_t epoch) {
-    int prefix = 0;
-    for (int k = tile - 1; k >= 0; --k) {
-        uint64_t w;
-        for (;;) {
-            w = __hip_atomic_load(&status[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t e = static_cast<uint32_t>(w >> 34);
-            const uint32_t f = static_cast<uint32_t>(w >> 32) & 3u;
-            if (e == (epoch & kEpochMask) && f != 0u) break;
-            __builtin_amdgcn_s_sleep(1);
-        }
-        prefix += static_cast<int>(static_cast<uint32_t>(w));
-        if (((w >> 32) & 3u) == kFlagPrefix) break;
-    }
-    return prefix;
-}
 
 // Inclusive scan across the 64 lanes of a wave.
 __device__ __forceinline__ int wave_inclusive(int x, int lane) {
@@ -77,105 +54,121 @@ __device__ __forceinline__ int wave_inclusive(int x, int lane) {
     return x;
 }
 
-// Generic tile scan; Op supplies value(e) (an int), emit(e, v, excl) and
-// finish(total) (called once, by the last tile, with the grand total).
-template <int ITEMS, bool BINARY, class Op>
-__global__ __launch_bounds__(kBlock) void scan_tiles_kernel(Op op, int n, ScanWs ws,
-                                                            int ntiles) {
-    __shared__ int s_tile;
-    __shared__ uint32_t s_epoch;
-    __shared__ int s_wave_tot[ITEMS][kWaves];
-    __shared__ int s_prefix;
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
 
+// Sum of x over the workgroup (every thread gets it).
+__device__ __forceinline__ int block_sum(int x, int *s_red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    x = wave_sum(x);
+    if (lane == 0) s_red[wave] = x;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += s_red[w];
+    __syncthreads();
+    return t;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void count_kernel(Op op, int n, int per_chunk,
+                                                       int *counts) {
+    __shared__ int s_red[kWaves];
+    const int b = blockIdx.x * per_chunk;
+    const int e_end = min(b + per_chunk, n);
+    int c = 0;
+#pragma unroll 8
+    for (int e = b + static_cast<int>(threadIdx.x); e < e_end; e += kBlock) c += op.value_at(e);
+    c = block_sum(c, s_red);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Op supplies value_at(e) (the int summed), an Item type with load(e) (every
+// field the element needs, so all of a tile's loads are in flight before any
+// store), value(item), emit(e, item, v, excl), and finish(total, header)
+// (called once, by the last chunk, with the grand total).
+template <bool BINARY, class Op>
+__global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int per_chunk,
+                                                             ScanWs ws) {
+    __shared__ int s_red[kWaves];
+    __shared__ int s_wave_tot[kItems][kWaves];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    if (t == 0) {
-        const uint32_t ep = __hip_atomic_load(&ws.header[1], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t id = __hip_atomic_fetch_add(&ws.header[0], 1u, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-        s_tile = static_cast<int>(id);
-        s_epoch = ep;
-    }
-    __syncthreads();
-    const int tile = s_tile;
-    const uint32_t epoch = s_epoch;
-    const int base = tile * (kBlock * ITEMS);
+    const int g = blockIdx.x;
 
-    int v[ITEMS], incl[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const int e = base + k * kBlock + t;
-        v[k] = e < n ? op.value(e) : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        int x;
-        if constexpr (BINARY) {
-            const uint64_t b = __ballot(v[k] != 0);
-            const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-            x = __popcll(b & le);
-        } else {
-            x = wave_inclusive(v[k], lane);
-        }
-        incl[k] = x;
-        if (lane == 63) s_wave_tot[k][wave] = x;
-    }
-    __syncthreads();
-    int running = 0;
-    int excl[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        int before = 0, slot = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            const int c = s_wave_tot[k][w];
-            before += (w < wave) ? c : 0;
-            slot += c;
-        }
-        excl[k] = running + before + incl[k] - v[k];
-        running += slot;
-    }
-    const int tile_total = running;
+    int offset = 0;
+    for (int j = t; j < g; j += kBlock) offset += ws.counts[j];
+    offset = block_sum(offset, s_red);
 
-    if (t == 0) {
-        int prefix = 0;
-        if (tile == 0) {
-            publish(&ws.status[0], make_status(epoch, kFlagPrefix, static_cast<uint32_t>(tile_total)));
-        } else {
-            publish(&ws.status[tile], make_status(epoch, kFlagAggregate, static_cast<uint32_t>(tile_total)));
-            prefix = look_back(ws.status, tile, epoch);
-            publish(&ws.status[tile],
-                    make_status(epoch, kFlagPrefix, static_cast<uint32_t>(prefix + tile_total)));
-        }
-        s_prefix = prefix;
-        if (tile == ntiles - 1) {
-            op.finish(prefix + tile_total, ws.header);
-            // Every tile took its index before this one did, and read the
-            // epoch before that: safe to recycle the counter and epoch.
-            __hip_atomic_store(&ws.header[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ws.header[1], (epoch + 1u) & kEpochMask, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    __syncthreads();
-    const int prefix = s_prefix;
+    const int c_end = min((g + 1) * per_chunk, n);
+    for (int base = g * per_chunk; base < c_end; base += kTile) {
+        typename Op::Item it[kItems];
+        int v[kItems], incl[kItems];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const int e = base + k * kBlock + t;
-        if (e < n) op.emit(e, v[k], prefix + excl[k]);
+        for (int k = 0; k < kItems; ++k) {
+            const int e = base + k * kBlock + t;
+            if (e < c_end) it[k] = op.load(e);
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const int e = base + k * kBlock + t;
+            v[k] = e < c_end ? op.value(it[k]) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            int x;
+            if constexpr (BINARY) {
+                const uint64_t bal = __ballot(v[k] != 0);
+                const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+                x = __popcll(bal & le);
+            } else {
+                x = wave_inclusive(v[k], lane);
+            }
+            incl[k] = x;
+            if (lane == 63) s_wave_tot[k][wave] = x;
+        }
+        __syncthreads();
+        int running = 0;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            int before = 0, slot = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const int c = s_wave_tot[k][w];
+                before += (w < wave) ? c : 0;
+                slot += c;
+            }
+            const int e = base + k * kBlock + t;
+            if (e < c_end) op.emit(e, it[k], v[k], offset + running + before + incl[k] - v[k]);
+            running += slot;
+        }
+        offset += running;
+        __syncthreads();  // s_wave_tot is rewritten by the next tile
     }
+    if (g == static_cast<int>(gridDim.x) - 1 && t == 0) op.finish(offset, ws.header);
 }
 
 // ------------------------------------------------------------------ ops
 
 struct ScanI32Op {
-    const int *in;
-    int *out;
-    __device__ int value(int e) const { return in[e]; }
-    __device__ void emit(int e, int, int excl) const { out[e] = excl; }
+    const int *__restrict__ in;
+    int *__restrict__ out;
+    using Item = int;
+    __device__ int value_at(int e) const { return in[e]; }
+    __device__ Item load(int e) const { return in[e]; }
+    __device__ int value(Item x) const { return x; }
+    __device__ void emit(int e, Item, int, int excl) const { out[e] = excl; }
     __device__ void finish(int, uint32_t *) const {}
+};
+
+// One particle record as read by the split (68 B in FP64).
+template <typename T> struct SplitRec {
+    T px, py, pz, vx, vy, vz, m, ts;
+    int id;
 };
 
 template <typename T> struct SplitOp {
@@ -184,22 +177,37 @@ template <typename T> struct SplitOp {
     T pivot;
     int n;
     hhs_partition *part;
+    using Item = SplitRec<T>;
     // src/split.cuh:115 -- NaN compares false and counts as slow.
-    __device__ int value(int e) const { return total.timestep[e] <= pivot ? 1 : 0; }
-    __device__ void emit(int e, int p, int f) const {
+    __device__ int value_at(int e) const { return total.timestep[e] <= pivot ? 1 : 0; }
+    __device__ Item load(int e) const {
+        Item r;
+        r.ts = total.timestep[e];
+        r.px = total.pos[3 * e + 0];
+        r.py = total.pos[3 * e + 1];
+        r.pz = total.pos[3 * e + 2];
+        r.vx = total.vel[3 * e + 0];
+        r.vy = total.vel[3 * e + 1];
+        r.vz = total.vel[3 * e + 2];
+        r.m = total.m[e];
+        r.id = total.id ? total.id[e] : e;
+        return r;
+    }
+    __device__ int value(const Item &r) const { return r.ts <= pivot ? 1 : 0; }
+    __device__ void emit(int e, const Item &r, int p, int f) const {
         if (predicate) predicate[e] = p;
         if (scanned) scanned[e] = f;
         const Particles<T> &d = p ? fast : slow;
         const int o = p ? f : e - f;  // src/split.cuh:336,350
-        d.pos[3 * o + 0] = total.pos[3 * e + 0];
-        d.pos[3 * o + 1] = total.pos[3 * e + 1];
-        d.pos[3 * o + 2] = total.pos[3 * e + 2];
-        d.vel[3 * o + 0] = total.vel[3 * e + 0];
-        d.vel[3 * o + 1] = total.vel[3 * e + 1];
-        d.vel[3 * o + 2] = total.vel[3 * e + 2];
-        d.m[o] = total.m[e];
-        d.timestep[o] = total.timestep[e];
-        if (d.id) d.id[o] = total.id ? total.id[e] : e;
+        d.pos[3 * o + 0] = r.px;
+        d.pos[3 * o + 1] = r.py;
+        d.pos[3 * o + 2] = r.pz;
+        d.vel[3 * o + 0] = r.vx;
+        d.vel[3 * o + 1] = r.vy;
+        d.vel[3 * o + 2] = r.vz;
+        d.m[o] = r.m;
+        d.timestep[o] = r.ts;
+        if (d.id) d.id[o] = r.id;
         d.parent_id[o] = e;
     }
     __device__ void finish(int nf, uint32_t *header) const {
@@ -211,17 +219,20 @@ template <typename T> struct SplitOp {
 };
 
 template <typename T> struct SplitValuesOp {
-    const T *values;
+    const T *__restrict__ values;
     T *slow, *fast;
     T pivot;
     int n;
     hhs_partition *part;
-    __device__ int value(int e) const { return values[e] <= pivot ? 1 : 0; }
-    __device__ void emit(int e, int p, int f) const {
+    using Item = T;
+    __device__ int value_at(int e) const { return values[e] <= pivot ? 1 : 0; }
+    __device__ Item load(int e) const { return values[e]; }
+    __device__ int value(Item x) const { return x <= pivot ? 1 : 0; }
+    __device__ void emit(int e, Item x, int p, int f) const {
         if (p)
-            fast[f] = values[e];
+            fast[f] = x;
         else
-            slow[e - f] = values[e];
+            slow[e - f] = x;
     }
     __device__ void finish(int nf, uint32_t *header) const {
         header[2] = static_cast<uint32_t>(nf);
@@ -237,18 +248,18 @@ __global__ __launch_bounds__(256) void lone_fast_fixup_kernel(Particles<T> total
                                                               Particles<T> slow, int n,
                                                               const uint32_t *header) {
     if (header[2] != 1u) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    slow.pos[3 * e + 0] = total.pos[3 * e + 0];
-    slow.pos[3 * e + 1] = total.pos[3 * e + 1];
-    slow.pos[3 * e + 2] = total.pos[3 * e + 2];
-    slow.vel[3 * e + 0] = total.vel[3 * e + 0];
-    slow.vel[3 * e + 1] = total.vel[3 * e + 1];
-    slow.vel[3 * e + 2] = total.vel[3 * e + 2];
-    slow.m[e] = total.m[e];
-    slow.timestep[e] = total.timestep[e];
-    if (slow.id) slow.id[e] = total.id ? total.id[e] : e;
-    slow.parent_id[e] = e;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        slow.pos[3 * e + 0] = total.pos[3 * e + 0];
+        slow.pos[3 * e + 1] = total.pos[3 * e + 1];
+        slow.pos[3 * e + 2] = total.pos[3 * e + 2];
+        slow.vel[3 * e + 0] = total.vel[3 * e + 0];
+        slow.vel[3 * e + 1] = total.vel[3 * e + 1];
+        slow.vel[3 * e + 2] = total.vel[3 * e + 2];
+        slow.m[e] = total.m[e];
+        slow.timestep[e] = total.timestep[e];
+        if (slow.id) slow.id[e] = total.id ? total.id[e] : e;
+        slow.parent_id[e] = e;
+    }
 }
 
 __global__ void zero_partition_kernel(hhs_partition *part) {
@@ -261,42 +272,39 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t words, uint32_t v) {
     if (i < words) p[i] = v;
 }
 
-int items_for(int n) {
-    // Small inputs: short tiles so several workgroups share the work; large
-    // inputs: 4 items per lane to amortise the look-back.
-    if (n <= 64 * 1024) return 1;
-    if (n <= 512 * 1024) return 2;
-    return 4;
+// Chunking: inputs of at most two tiles take one workgroup (one launch);
+// larger ones are cut into at most kMaxChunks chunks of whole tiles.
+struct Chunks {
+    int count, per_chunk;
+};
+Chunks chunks_for(int n) {
+    const int tiles = (n + kTile - 1) / kTile;
+    if (tiles <= 2) return {1, tiles * kTile};
+    const int g = tiles < kMaxChunks ? tiles : kMaxChunks;
+    const int tiles_per = (tiles + g - 1) / g;
+    return {(tiles + tiles_per - 1) / tiles_per, tiles_per * kTile};
 }
 
 template <bool BINARY, class Op>
 int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream_t st) {
-    const int items = items_for(n);
-    const int tile = kBlock * items;
-    const int ntiles = (n + tile - 1) / tile;
+    const Chunks c = chunks_for(n);
     HHS_REQUIRE(ws_ptr, "scan: null workspace");
-    if (ws_bytes < kScanHeaderBytes + sizeof(uint64_t) * static_cast<size_t>(ntiles)) {
+    if (ws_bytes < kScanHeaderBytes + sizeof(int) * static_cast<size_t>(c.count) ||
+        ws_bytes < scan_ws_bytes(n)) {
         set_error("scan: workspace of %zu bytes too small for n=%d (need %zu)", ws_bytes, n,
                   scan_ws_bytes(n));
         return HHS_ERR_WORKSPACE;
     }
     ScanWs ws{static_cast<uint32_t *>(ws_ptr),
-              reinterpret_cast<uint64_t *>(static_cast<char *>(ws_ptr) + kScanHeaderBytes)};
-    switch (items) {
-    case 1:
-        hipLaunchKernelGGL((scan_tiles_kernel<1, BINARY, Op>), dim3(ntiles), dim3(kBlock), 0, st,
-                           op, n, ws, ntiles);
-        break;
-    case 2:
-        hipLaunchKernelGGL((scan_tiles_kernel<2, BINARY, Op>), dim3(ntiles), dim3(kBlock), 0, st,
-                           op, n, ws, ntiles);
-        break;
-    default:
-        hipLaunchKernelGGL((scan_tiles_kernel<4, BINARY, Op>), dim3(ntiles), dim3(kBlock), 0, st,
-                           op, n, ws, ntiles);
-        break;
+              reinterpret_cast<int *>(static_cast<char *>(ws_ptr) + kScanHeaderBytes)};
+    if (c.count > 1) {
+        hipLaunchKernelGGL((count_kernel<Op>), dim3(c.count), dim3(kBlock), 0, st, op, n,
+                           c.per_chunk, ws.counts);
+        HHS_LAUNCH_CHECK("count_kernel");
     }
-    HHS_LAUNCH_CHECK("scan_tiles_kernel");
+    hipLaunchKernelGGL((scan_chunks_kernel<BINARY, Op>), dim3(c.count), dim3(kBlock), 0, st, op,
+                       n, c.per_chunk, ws);
+    HHS_LAUNCH_CHECK("scan_chunks_kernel");
     return HHS_OK;
 }
 
@@ -336,8 +344,9 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
     SplitOp<T> op{total, slow, fast, predicate, scanned, pivot, n, part_dev};
     int rc = launch_scan_op<true>(op, n, ws, ws_bytes, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((lone_fast_fixup_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, st,
-                       total, slow, n, static_cast<const uint32_t *>(ws));
+    const int fix_groups = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
+    hipLaunchKernelGGL((lone_fast_fixup_kernel<T>), dim3(fix_groups), dim3(256), 0, st, total,
+                       slow, n, static_cast<const uint32_t *>(ws));
     HHS_LAUNCH_CHECK("lone_fast_fixup_kernel");
     return HHS_OK;
 }

@@ -26,15 +26,20 @@ __global__ __launch_bounds__(256) void update_kernel(Particles<T> prev, Particle
                                                      int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int p = cur.parent_id[i];  // src/update_system.cuh:14-22
-    prev.pos[3 * p + 0] = cur.pos[3 * i + 0];
-    prev.pos[3 * p + 1] = cur.pos[3 * i + 1];
-    prev.pos[3 * p + 2] = cur.pos[3 * i + 2];
-    prev.vel[3 * p + 0] = cur.vel[3 * i + 0];
-    prev.vel[3 * p + 1] = cur.vel[3 * i + 1];
-    prev.vel[3 * p + 2] = cur.vel[3 * i + 2];
-    prev.m[p] = cur.m[i];
-    prev.timestep[p] = cur.timestep[i];
+    // src/update_system.cuh:14-22.  All loads first: prev and cur may not
+    // be proven disjoint, and interleaved stores would serialise the loads.
+    const int p = cur.parent_id[i];
+    const T x = cur.pos[3 * i + 0], y = cur.pos[3 * i + 1], z = cur.pos[3 * i + 2];
+    const T vx = cur.vel[3 * i + 0], vy = cur.vel[3 * i + 1], vz = cur.vel[3 * i + 2];
+    const T m = cur.m[i], ts = cur.timestep[i];
+    prev.pos[3 * p + 0] = x;
+    prev.pos[3 * p + 1] = y;
+    prev.pos[3 * p + 2] = z;
+    prev.vel[3 * p + 0] = vx;
+    prev.vel[3 * p + 1] = vy;
+    prev.vel[3 * p + 2] = vz;
+    prev.m[p] = m;
+    prev.timestep[p] = ts;
 }
 
 // Row sums of the potential: row[i] = m_i * sum_{j>i} m_j / sqrt(r^2+eps2),

@@ -12,7 +12,7 @@ for sub in ("parallel-hhs_amd", "oracle"):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: long CPU oracle run")
 
 
