@@ -288,16 +288,16 @@ int size_class(int n) { return n <= 1 ? 0 : 31 - __builtin_clz(static_cast<unsig
 
 }  // namespace
 
-// Heuristic from the MI355X sweep at N = 131072 (FP64 storage): W = 8 with
-// S = 1 or 2 reaches 2606 Ginteractions/s (~15 VALU ops per interaction at
-// the full issue rate), S = 4 / W = 4 2164, S = 4 / W = 8 2012.  Latency is
-// hidden by waves, not by deeper register blocking: use up to 8 waves per
-// workgroup (8 waves per SIMD at N = 131072) while each wave keeps >= 64
-// sources, and 2 sinks per lane only for large sink counts.
+// Heuristic from the MI355X sweep (FP64 storage, MIXED32): at N = 131072,
+// S = 1 / W = 8 gives 2596 Ginteractions/s, S = 2 / W = 8 2566, S = 1 /
+// W = 4 2377, S = 4 / W = 4 2150; at N = 16384, S = 1 / W = 8 1413 and
+// S = 2 / W = 8 895.  Latency is hidden by waves, not by deeper register
+// blocking: one sink per lane and up to 8 waves per workgroup while each
+// wave keeps >= 64 sources.
 KickConfig kick_pick_config(int n_sink, int n_src) {
     KickConfig c;
     const int fs = g_force_s.load(), fw = g_force_w.load();
-    c.sinks_per_lane = n_sink >= 65536 ? 2 : 1;
+    c.sinks_per_lane = 1;
     int w = 1;
     while (w < 8 && n_src / (2 * w) >= 64) w *= 2;
     c.waves = w;

@@ -77,13 +77,13 @@ def test_invalid_arguments_return_status(hhs):
 
 def test_kick_config_heuristic(hhs):
     s, w = hhs.kick_get_config(131072, 131072)
-    assert (s, w) == (2, 8)
+    assert (s, w) == (1, 8)
     s, w = hhs.kick_get_config(100, 100)
     assert s == 1 and 1 <= w <= 8
     hhs.kick_set_config(4, 2)
     assert hhs.kick_get_config(131072, 131072) == (4, 2)
     hhs.kick_set_config(0, 0)
-    assert hhs.kick_get_config(131072, 131072) == (2, 8)
+    assert hhs.kick_get_config(131072, 131072) == (1, 8)
 
 
 # ------------------------------------------------------ initial conditions
@@ -138,7 +138,7 @@ def test_autotune_cache_round_trip(hhs, tmp_path):
         hhs.kick_set_config(0, 0)
     finally:
         hhs.autotune_clear()
-    assert hhs.kick_get_config(131072, 131072) == (2, 8)
+    assert hhs.kick_get_config(131072, 131072) == (1, 8)
     bad = tmp_path / "bad.txt"
     bad.write_text("nothing here\n")
     with pytest.raises(hhs.HHSError):
