@@ -159,9 +159,37 @@ __device__ __forceinline__ void copy_rec(const EnsArgs &a, bool to_tmp, int src,
     di[dst] = si[src];
 }
 
-// timesteps of particles [o+b, o+e) against each other (src/timestep.cuh).
+// Source tiles: the team copies TEAM sources at a time into its LDS tile
+// (one coalesced load per thread), then every lane reads them as LDS
+// broadcasts -- one L2 round trip per tile instead of one per source.
+// Timestep tiles hold x, y, z, m, vx, vy, vz (stride kTsStride doubles);
+// kick tiles hold x, y, z, m (stride kKickStride).
+constexpr int kTsStride = 8;
+constexpr int kKickStride = 4;
+
 template <int TEAM>
-__device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, const Team<TEAM> &tm) {
+__device__ __forceinline__ void load_ts_tile(const EnsArgs &a, int s0, int je, double *tile,
+                                             const Team<TEAM> &tm) {
+    tm.sync();  // the previous tile has been read by every lane
+    const int j = s0 + tm.t;
+    if (j < je) {
+        double *q = tile + kTsStride * tm.t;
+        q[0] = a.wpos[3 * j];
+        q[1] = a.wpos[3 * j + 1];
+        q[2] = a.wpos[3 * j + 2];
+        q[3] = a.wm[j];
+        q[4] = a.wvel[3 * j];
+        q[5] = a.wvel[3 * j + 1];
+        q[6] = a.wvel[3 * j + 2];
+    }
+    tm.sync();
+}
+
+// timesteps of particles [o+b, o+e) against each other (src/timestep.cuh),
+// sources in index order through LDS tiles.
+template <int TEAM>
+__device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, double *tile,
+                               const Team<TEAM> &tm) {
     const int jb = uni(o + b), je = uni(o + e);
     for (int i0 = jb; i0 < je; i0 += TEAM) {
         const int i = i0 + tm.t;
@@ -169,33 +197,43 @@ __device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, const Team
         const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
         const double vxi = a.wvel[3 * ic], vyi = a.wvel[3 * ic + 1], vzi = a.wvel[3 * ic + 2];
         const double mi = a.wm[ic];
-        double minval = TsConst<double>::huge;
-        if (a.ts_mode == HHS_TIMESTEP_FAST) {
-            double bden = 1.0;
-            for (int j = jb; j < je; ++j)
-                pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
-                                   a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
-                                   a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta, a.c_ff,
-                                   a.eps2, j != i, minval, bden);
-            minval = minval / bden;
-        } else if (a.ts_mode == HHS_TIMESTEP_FP32) {
-            float bnum = static_cast<float>(TsConst<double>::huge), bden = 1.0f;
-            const float mif = static_cast<float>(mi), etaf = static_cast<float>(a.eta),
-                        cff = static_cast<float>(a.c_ff);
-            for (int j = jb; j < je; ++j)
-                pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mif, a.wpos[3 * j],
-                                  a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
-                                  a.wvel[3 * j + 1], a.wvel[3 * j + 2],
-                                  static_cast<float>(a.wm[j]), etaf, cff, a.eps2f, j != i, bnum,
-                                  bden);
-            minval = static_cast<double>(bnum) / static_cast<double>(bden);
-        } else {
-            for (int j = jb; j < je; ++j)
-                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
-                                      a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
-                                      a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta,
-                                      a.c_ff, a.eps2, j != i, minval);
+        double minval = TsConst<double>::huge, bden = 1.0;
+        float bnum32 = static_cast<float>(TsConst<double>::huge), bden32 = 1.0f;
+        const float mif = static_cast<float>(mi), etaf = static_cast<float>(a.eta),
+                    cff = static_cast<float>(a.c_ff);
+        for (int s0 = jb; s0 < je; s0 += TEAM) {
+            load_ts_tile<TEAM>(a, s0, je, tile, tm);
+            const int cnt = uni(min(TEAM, je - s0));
+            if (a.ts_mode == HHS_TIMESTEP_FP32) {
+#pragma unroll 2
+                for (int k = 0; k < cnt; ++k) {
+                    const double *q = tile + kTsStride * k;
+                    pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mif, q[0], q[1], q[2], q[4], q[5],
+                                      q[6], static_cast<float>(q[3]), etaf, cff, a.eps2f,
+                                      s0 + k != i, bnum32, bden32);
+                }
+            } else if (a.ts_mode == HHS_TIMESTEP_FAST) {
+#pragma unroll 2
+                for (int k = 0; k < cnt; ++k) {
+                    const double *q = tile + kTsStride * k;
+                    pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, q[0], q[1], q[2], q[4], q[5],
+                                       q[6], q[3], a.eta, a.c_ff, a.eps2, s0 + k != i, minval,
+                                       bden);
+                }
+            } else {
+#pragma unroll 2
+                for (int k = 0; k < cnt; ++k) {
+                    const double *q = tile + kTsStride * k;
+                    pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, q[0], q[1], q[2], q[4],
+                                          q[5], q[6], q[3], a.eta, a.c_ff, a.eps2, s0 + k != i,
+                                          minval);
+                }
+            }
         }
+        if (a.ts_mode == HHS_TIMESTEP_FP32)
+            minval = static_cast<double>(bnum32) / static_cast<double>(bden32);
+        else if (a.ts_mode == HHS_TIMESTEP_FAST)
+            minval = minval / bden;
         if (i < je) a.wts[i] = minval;
     }
 }
@@ -238,10 +276,14 @@ __device__ void team_drift(const EnsArgs &a, int o, int b, int e, double h,
 }
 
 // vel[i] += h * sum_j m_j (x_j - x_i) / (r^2 + eps2)^{3/2}, sinks [sb, se),
-// sources [jb, je) (absolute indices).
+// sources [jb, je) (absolute indices) in index order through LDS tiles,
+// summed in FP32 over blocks of kFold sources from jb (TEAM is a multiple of
+// kFold, so tile edges fall on block edges) folded into a two-sum
+// accumulator.
 template <int TEAM, bool SOFT>
 __device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, double h,
-                          const Team<TEAM> &tm) {
+                          double *tile, const Team<TEAM> &tm) {
+    static_assert(TEAM % kFold == 0, "tiles must hold whole fold blocks");
     jb = uni(jb);
     je = uni(je);
     for (int i0 = uni(sb); i0 < se; i0 += TEAM) {
@@ -249,16 +291,31 @@ __device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, doub
         const int ic = i < se ? i : se - 1;
         const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
         float hx = 0.f, lx = 0.f, hy = 0.f, ly = 0.f, hz = 0.f, lz = 0.f;
-        for (int j0 = jb; j0 < je; j0 += kFold) {
-            const int j1 = min(j0 + kFold, je);
-            float px = 0.f, py = 0.f, pz = 0.f;
+        for (int s0 = jb; s0 < je; s0 += TEAM) {
+            tm.sync();  // the previous tile has been read by every lane
+            const int j = s0 + tm.t;
+            if (j < je) {
+                double *q = tile + kKickStride * tm.t;
+                q[0] = a.wpos[3 * j];
+                q[1] = a.wpos[3 * j + 1];
+                q[2] = a.wpos[3 * j + 2];
+                q[3] = a.wm[j];
+            }
+            tm.sync();
+            const int cnt = uni(min(TEAM, je - s0));
+            for (int k0 = 0; k0 < cnt; k0 += kFold) {
+                const int k1 = min(k0 + kFold, cnt);
+                float px = 0.f, py = 0.f, pz = 0.f;
 #pragma unroll 4
-            for (int j = j0; j < j1; ++j)
-                pair_accel<SOFT>(a.wpos[3 * j], a.wpos[3 * j + 1], a.wpos[3 * j + 2],
-                                 static_cast<float>(a.wm[j]), xi, yi, zi, a.eps2f, px, py, pz);
-            two_sum_add(hx, lx, px);
-            two_sum_add(hy, ly, py);
-            two_sum_add(hz, lz, pz);
+                for (int k = k0; k < k1; ++k) {
+                    const double *q = tile + kKickStride * k;
+                    pair_accel<SOFT>(q[0], q[1], q[2], static_cast<float>(q[3]), xi, yi, zi,
+                                     a.eps2f, px, py, pz);
+                }
+                two_sum_add(hx, lx, px);
+                two_sum_add(hy, ly, py);
+                two_sum_add(hz, lz, pz);
+            }
         }
         if (i < se) {
             const double ax = static_cast<double>(hx) + static_cast<double>(lx);
@@ -314,7 +371,7 @@ struct SysStats {
 // an explicit stack).  Returns HHS_OK or a status code (uniform).
 template <int TEAM, bool SOFT>
 __device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamShared &sh,
-                              const Team<TEAM> &tm, SysStats &st) {
+                              double *tile, const Team<TEAM> &tm, SysStats &st) {
     if (tm.t == 0) {
         sh.sp = 1;
         sh.status = HHS_OK;
@@ -336,7 +393,7 @@ __device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamSha
             st.calls++;
             st.max_level = max(st.max_level, level);
             if (f.calc) {  // src/hold.cuh:74-76
-                team_timesteps<TEAM>(a, o, f.b, f.e, tm);
+                team_timesteps<TEAM>(a, o, f.b, f.e, tile, tm);
                 st.ts_pairs += static_cast<long long>(f.e - f.b) * (f.e - f.b - 1);
                 tm.sync();
             }
@@ -356,8 +413,9 @@ __device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamSha
                 team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);  // :123-124
                 tm.sync();
                 // :137-145 -- slow sinks from every particle, fast sinks from slow ones
-                team_kick<TEAM, SOFT>(a, o + sb, o + f.e, o + f.b, o + f.e, f.dt, tm);
-                if (f.nf > 0) team_kick<TEAM, SOFT>(a, o + f.b, o + sb, o + sb, o + f.e, f.dt, tm);
+                team_kick<TEAM, SOFT>(a, o + sb, o + f.e, o + f.b, o + f.e, f.dt, tile, tm);
+                if (f.nf > 0)
+                    team_kick<TEAM, SOFT>(a, o + f.b, o + sb, o + sb, o + f.e, f.dt, tile, tm);
                 st.interactions += static_cast<long long>(f.e - sb) * (f.e - f.b) +
                                    static_cast<long long>(f.nf) * (f.e - sb);
                 tm.sync();
@@ -383,7 +441,8 @@ __device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamSha
 }
 
 template <int TEAM, bool SOFT>
-__device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, const Team<TEAM> &tm) {
+__device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double *tile,
+                                const Team<TEAM> &tm) {
     const int o = uni(a.offsets[s]);
     const int n = uni(a.offsets[s + 1]) - o;
     for (int i = tm.t; i < n; i += TEAM) {
@@ -406,7 +465,7 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, const T
         const double t_end = a.t_end[s], dtr = a.dt_root[s];
         if (!(dtr > 0.0)) status = HHS_ERR_INVALID;
         while (status == HHS_OK && t < t_end) {
-            status = team_hold_root<TEAM, SOFT>(a, o, n, dtr, sh, tm, st);
+            status = team_hold_root<TEAM, SOFT>(a, o, n, dtr, sh, tile, tm, st);
             if (status == HHS_OK) t += dtr;
         }
         e1 = team_energy<TEAM>(a, o, n, sh, tm);
@@ -439,6 +498,9 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, const T
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
     __shared__ TeamShared sh[4];
+    // Source tiles: the whole array for a workgroup team (kWG sources), a
+    // quarter per wave team (64 sources each).
+    __shared__ double tiles[kWG * kTsStride];
     __shared__ int s_item;
     // Big systems: the whole workgroup on one system.
     for (;;) {
@@ -450,7 +512,7 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
         __syncthreads();
         if (k >= a.n_big) break;
         Team<kWG> tm;
-        team_run_system<kWG, SOFT>(a, a.big_list[k], sh[0], tm);
+        team_run_system<kWG, SOFT>(a, a.big_list[k], sh[0], tiles, tm);
     }
     // Small systems: each wave pulls its own.
     const int w = threadIdx.x >> 6;
@@ -462,7 +524,7 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
                                                         __HIP_MEMORY_SCOPE_AGENT));
         k = __shfl(k, 0, 64);
         if (k >= a.n_small) break;
-        team_run_system<64, SOFT>(a, a.small_list[k], sh[w], tw);
+        team_run_system<64, SOFT>(a, a.small_list[k], sh[w], tiles + w * 64 * kTsStride, tw);
     }
 }
 
