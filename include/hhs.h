@@ -283,6 +283,13 @@ int hhs_globals_set_timestep_mode(hhs_globals *g, int mode);
  * recursion is bit-exact with the FP64 CPU HOLD, so the whole integration
  * is.  No reference counterpart (the reference has one arithmetic per T). */
 int hhs_globals_set_kick_precision(hhs_globals *g, int mode);
+/* Calls of hold_step with at most n_max particles run their whole
+ * recursion subtree in one device launch (one workgroup; MIXED32 kicks and
+ * the globals' timestep mode), with one host sync per subtree instead of
+ * one per call.  0 disables it.  Used only when the kick precision in force
+ * is HHS_KICK_MIXED32.  Default 256.  No reference counterpart. */
+int hhs_globals_set_subtree_max(hhs_globals *g, int n_max);
+int hhs_globals_subtree_max(const hhs_globals *g);
 int hhs_globals_timestep_mode(const hhs_globals *g);
 int hhs_globals_kick_precision(const hhs_globals *g);
 /* Root steps completed by hhs_evolve_f64 (restored from a snapshot on

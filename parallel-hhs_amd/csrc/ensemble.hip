@@ -31,7 +31,7 @@ namespace hhs {
 namespace {
 
 constexpr int kWG = 256;
-constexpr int kMaxFrames = 50;  // max_level <= 47 plus root and one child
+constexpr int kMaxFrames = kSubtreeLevels;  // max_level <= 47 plus root and one child
 constexpr int kFold = 64;
 
 struct Frame {
@@ -42,6 +42,7 @@ struct Frame {
 
 struct TeamShared {
     Frame stack[kMaxFrames];
+    int level_calls[kMaxFrames];  // calls per recursion level (relative)
     int sp;
     int status;
     int wave_tot[4];
@@ -367,15 +368,18 @@ struct SysStats {
     int max_level = 0;
 };
 
-// One HOLD root step of dt over the whole system (src/hold.cuh:51-181 with
-// an explicit stack).  Returns HHS_OK or a status code (uniform).
+// hold_step (src/hold.cuh:51-181) on particles [o, o+n) with time step dt
+// and the given calc_timestep flag, run to completion with an explicit
+// stack: a root step is (dt_root, calc = 1).  Levels are counted from 0 at
+// this call.  Returns HHS_OK or a status code (uniform).
 template <int TEAM, bool SOFT>
-__device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamShared &sh,
-                              double *tile, const Team<TEAM> &tm, SysStats &st) {
+__device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int calc,
+                               TeamShared &sh, double *tile, const Team<TEAM> &tm,
+                               SysStats &st) {
     if (tm.t == 0) {
         sh.sp = 1;
         sh.status = HHS_OK;
-        sh.stack[0] = Frame{0, n, 0, 0, dt, 1, 0};
+        sh.stack[0] = Frame{0, n, 0, 0, dt, calc, 0};
     }
     tm.sync();
     for (;;) {
@@ -392,6 +396,7 @@ __device__ int team_hold_root(const EnsArgs &a, int o, int n, double dt, TeamSha
             }
             st.calls++;
             st.max_level = max(st.max_level, level);
+            if (tm.t == 0) sh.level_calls[level]++;
             if (f.calc) {  // src/hold.cuh:74-76
                 team_timesteps<TEAM>(a, o, f.b, f.e, tile, tm);
                 st.ts_pairs += static_cast<long long>(f.e - f.b) * (f.e - f.b - 1);
@@ -456,6 +461,8 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
         a.wts[g] = 0.0;
         a.wid[g] = i;
     }
+    if (tm.t == 0)
+        for (int l = 0; l < kMaxFrames; ++l) sh.level_calls[l] = 0;
     tm.sync();
     SysStats st;
     int status = HHS_OK;
@@ -465,7 +472,7 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
         const double t_end = a.t_end[s], dtr = a.dt_root[s];
         if (!(dtr > 0.0)) status = HHS_ERR_INVALID;
         while (status == HHS_OK && t < t_end) {
-            status = team_hold_root<TEAM, SOFT>(a, o, n, dtr, sh, tile, tm, st);
+            status = team_hold_frame<TEAM, SOFT>(a, o, n, dtr, 1, sh, tile, tm, st);
             if (status == HHS_OK) t += dtr;
         }
         e1 = team_energy<TEAM>(a, o, n, sh, tm);
@@ -528,6 +535,52 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
     }
 }
 
+// One recursion subtree of a per-system HOLD, run by one workgroup: the
+// particles of the call are copied into the working arrays (original index
+// in wid), the engine above integrates the subtree in place, and the
+// records (pos, vel, m, timestep) go back to their original slots.  The
+// host pays one launch and one sync for the whole subtree instead of one
+// sync and ~8 launches per call.
+template <bool SOFT>
+__global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<double> total, int n,
+                                                      double dt, int calc, SubtreeStats *out) {
+    __shared__ TeamShared sh;
+    __shared__ double tile[kWG * kTsStride];
+    Team<kWG> tm;
+    for (int i = tm.t; i < n; i += kWG) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a.wpos[3 * i + k] = total.pos[3 * i + k];
+            a.wvel[3 * i + k] = total.vel[3 * i + k];
+        }
+        a.wm[i] = total.m[i];
+        a.wts[i] = total.timestep[i];
+        a.wid[i] = i;
+    }
+    for (int l = tm.t; l < kMaxFrames; l += kWG) sh.level_calls[l] = 0;
+    tm.sync();
+    SysStats st;
+    const int status = team_hold_frame<kWG, SOFT>(a, 0, n, dt, calc, sh, tile, tm, st);
+    for (int g = tm.t; g < n; g += kWG) {
+        const int u = a.wid[g];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            total.pos[3 * u + k] = a.wpos[3 * g + k];
+            total.vel[3 * u + k] = a.wvel[3 * g + k];
+        }
+        total.m[u] = a.wm[g];
+        total.timestep[u] = a.wts[g];
+    }
+    if (tm.t == 0) {
+        out->calls = st.calls;
+        out->interactions = st.interactions;
+        out->ts_pairs = st.ts_pairs;
+        out->max_level = st.max_level;
+        out->status = status;
+        for (int l = 0; l < kMaxFrames; ++l) out->level_calls[l] = sh.level_calls[l];
+    }
+}
+
 __global__ void zero_counters_kernel(unsigned int *c) {
     if (threadIdx.x < 16) c[threadIdx.x] = 0u;
 }
@@ -566,6 +619,49 @@ EnsLayout layout(int nsys, long long np) {
 }
 
 }  // namespace
+
+size_t subtree_workspace_bytes(int n_max) {
+    const EnsLayout L = layout(1, n_max);
+    return L.total;
+}
+
+int launch_subtree(const Particles<double> &total, int n, double dt, int calc, double eta,
+                   double eps2, int ts_mode, int max_level, void *ws, size_t ws_bytes,
+                   SubtreeStats *out_dev, hipStream_t st) {
+    HHS_REQUIRE(n > 0 && max_level >= 0 && max_level <= kMaxFrames - 3,
+                "subtree: n=%d, max_level=%d", n, max_level);
+    const EnsLayout L = layout(1, n);
+    HHS_REQUIRE(ws && ws_bytes >= L.total, "subtree: workspace of %zu bytes, need %zu",
+                ws_bytes, L.total);
+    char *w = static_cast<char *>(ws);
+    EnsArgs a{};
+    a.eta = eta;
+    a.c_ff = eta / TsConst<double>::sqrt2;
+    a.eps2 = eps2;
+    a.eps2f = static_cast<float>(eps2);
+    a.max_level = max_level;
+    a.max_calls = 0;
+    a.wpos = reinterpret_cast<double *>(w + L.wpos);
+    a.wvel = reinterpret_cast<double *>(w + L.wvel);
+    a.wm = reinterpret_cast<double *>(w + L.wm);
+    a.wts = reinterpret_cast<double *>(w + L.wts);
+    a.wid = reinterpret_cast<int *>(w + L.wid);
+    a.tpos = reinterpret_cast<double *>(w + L.tpos);
+    a.tvel = reinterpret_cast<double *>(w + L.tvel);
+    a.tm = reinterpret_cast<double *>(w + L.tm);
+    a.tts = reinterpret_cast<double *>(w + L.tts);
+    a.tid = reinterpret_cast<int *>(w + L.tid);
+    a.ts_mode = ts_mode;
+    if (eps2 > 0.0)
+        hipLaunchKernelGGL(subtree_kernel<true>, dim3(1), dim3(kWG), 0, st, a, total, n, dt, calc,
+                           out_dev);
+    else
+        hipLaunchKernelGGL(subtree_kernel<false>, dim3(1), dim3(kWG), 0, st, a, total, n, dt,
+                           calc, out_dev);
+    HHS_LAUNCH_CHECK("subtree_kernel");
+    return HHS_OK;
+}
+
 }  // namespace hhs
 
 using namespace hhs;

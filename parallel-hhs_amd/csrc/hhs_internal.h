@@ -106,4 +106,21 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
 template <typename T>
 int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
 
+// ------------------------------------------------- device-side subtrees
+// Statistics of one hold_step subtree run on the device (levels relative to
+// the subtree's first call).
+constexpr int kSubtreeLevels = 50;
+struct SubtreeStats {
+    long long calls, interactions, ts_pairs;
+    int max_level, status;
+    int level_calls[kSubtreeLevels];
+};
+size_t subtree_workspace_bytes(int n_max);
+// hold_step(total[0..n), dt, calc) with its whole recursion in one
+// workgroup-wide launch (MIXED32 kicks, timestep mode ts_mode); stats go to
+// out_dev.  max_level is relative to this call (<= kSubtreeLevels - 3).
+int launch_subtree(const Particles<double> &total, int n, double dt, int calc, double eta,
+                   double eps2, int ts_mode, int max_level, void *ws, size_t ws_bytes,
+                   SubtreeStats *out_dev, hipStream_t st);
+
 }  // namespace hhs

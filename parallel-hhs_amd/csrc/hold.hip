@@ -47,6 +47,13 @@ struct hhs_globals {
     int timestep_mode = HHS_TIMESTEP_EXACT;
     int kick_precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
     long long root_steps = 0; // root steps completed by hhs_evolve_f64
+    // Calls with at most subtree_max particles run their whole subtree in one
+    // device launch (0 = never).
+    int subtree_max = 0;
+    void *sub_ws = nullptr;
+    size_t sub_ws_bytes = 0;
+    hhs::SubtreeStats *sub_dev = nullptr;
+    hhs::SubtreeStats *sub_host = nullptr;  // pinned
 };
 
 namespace hhs {
@@ -94,6 +101,64 @@ int ensure_level(hhs_globals *g, int level) {
     return HHS_OK;
 }
 
+// Default size limit of device-run subtrees, from the MI355X sweep over
+// {0, 64, 256, 1024, 2048}: 256 was fastest at every size measured (N = 1024
+// integration 0.64 -> 0.20 s, N = 2048 1.08 -> 0.35 s, N = 8192 0.59 ->
+// 0.30 s, one N = 131072 root step 3.45 -> 3.02 s).  Larger subtrees lose
+// because one workgroup then does work the whole GPU would do faster.
+constexpr int kDefaultSubtreeMax = 256;
+
+// The device subtree runs the MIXED32 kick; other kick modes keep the host
+// recursion so that their arithmetic is the one requested.
+bool use_subtree(const hhs_globals *g, int n, int clevel) {
+    if (g->subtree_max <= 0 || n > g->subtree_max || !g->sub_ws) return false;
+    const int prec = g->kick_precision >= 0 ? g->kick_precision : hhs_kick_get_precision();
+    return prec == HHS_KICK_MIXED32 && g->max_level - clevel <= kSubtreeLevels - 3;
+}
+
+int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
+                bool calc, hipStream_t st) {
+    int rc = launch_subtree(total, n, dt, calc ? 1 : 0, g->eta, g->eps2, g->timestep_mode,
+                            g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_dev, st);
+    if (rc) return rc;
+    HHS_HIP(hipMemcpyAsync(g->sub_host, g->sub_dev, sizeof(SubtreeStats), hipMemcpyDeviceToHost,
+                           st));
+    HHS_HIP(hipStreamSynchronize(st));
+    const SubtreeStats &o = *g->sub_host;
+    hhs_hold_stats &S = g->stats;
+    S.calls += o.calls;
+    S.interactions += o.interactions;
+    S.ts_pairs += o.ts_pairs;
+    for (int l = 0; l < kSubtreeLevels && clevel + l < 64; ++l)
+        S.calls_per_level[clevel + l] += o.level_calls[l];
+    if (clevel + o.max_level > S.max_level) S.max_level = clevel + o.max_level;
+    if (o.status != HHS_OK) {
+        set_error("hold_step: recursion deeper than max_level %d (timesteps collapsed?)",
+                  g->max_level);
+        return o.status;
+    }
+    g->simtime += dt;  // the leaves of a completed subtree tile its dt
+    return HHS_OK;
+}
+
+int alloc_subtree(hhs_globals *g, int n_max) {
+    if (g->sub_ws) HHS_HIP(hipFree(g->sub_ws));
+    g->sub_ws = nullptr;
+    g->sub_ws_bytes = 0;
+    g->subtree_max = 0;
+    if (n_max <= 0) return HHS_OK;
+    const int cap = n_max < g->n_total ? n_max : g->n_total;
+    const size_t bytes = subtree_workspace_bytes(cap);
+    if (!g->sub_dev) HHS_HIP(hipMalloc(&g->sub_dev, sizeof(SubtreeStats)));
+    if (!g->sub_host)
+        HHS_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->sub_host), sizeof(SubtreeStats),
+                              hipHostMallocDefault));
+    HHS_HIP(hipMalloc(&g->sub_ws, bytes));
+    g->sub_ws_bytes = bytes;
+    g->subtree_max = cap;
+    return HHS_OK;
+}
+
 int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
              bool calc, hipStream_t st) {
     if (clevel > g->max_level) {
@@ -102,6 +167,7 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
         return HHS_ERR_DEPTH;
     }
     hhs_hold_stats &S = g->stats;
+    if (use_subtree(g, n, clevel)) return run_subtree(g, total, n, clevel, dt, calc, st);
     S.calls++;
     if (clevel < 64) S.calls_per_level[clevel]++;
     if (clevel > S.max_level) S.max_level = clevel;
@@ -207,6 +273,10 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
         hhs_globals_destroy(g);
         return rc;
     }
+    if (int rc = alloc_subtree(g, kDefaultSubtreeMax)) {
+        hhs_globals_destroy(g);
+        return rc;
+    }
     *out = g;
     return HHS_OK;
 }
@@ -220,6 +290,9 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->scan_ws) (void)hipFree(g->scan_ws);
     if (g->part_dev) (void)hipFree(g->part_dev);
     if (g->part_host) (void)hipHostFree(g->part_host);
+    if (g->sub_ws) (void)hipFree(g->sub_ws);
+    if (g->sub_dev) (void)hipFree(g->sub_dev);
+    if (g->sub_host) (void)hipHostFree(g->sub_host);
     delete g;
     return HHS_OK;
 }
@@ -249,6 +322,14 @@ int hhs_globals_set_kick_precision(hhs_globals *g, int mode) {
     g->kick_precision = mode;
     return HHS_OK;
 }
+
+int hhs_globals_set_subtree_max(hhs_globals *g, int n_max) {
+    HHS_REQUIRE(g, "set_subtree_max: null globals");
+    HHS_REQUIRE(n_max >= 0, "set_subtree_max: n_max must be >= 0");
+    HHS_HIP(hipDeviceSynchronize());  // the old workspace may still be in use
+    return alloc_subtree(g, n_max);
+}
+int hhs_globals_subtree_max(const hhs_globals *g) { return g ? g->subtree_max : 0; }
 
 int hhs_globals_timestep_mode(const hhs_globals *g) { return g ? g->timestep_mode : -1; }
 int hhs_globals_kick_precision(const hhs_globals *g) { return g ? g->kick_precision : -1; }

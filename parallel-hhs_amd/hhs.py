@@ -154,6 +154,8 @@ _SIGS = {
     "hhs_timesteps_mode_f64": (_i, [_P64, _i, _d, _d, _i, _vp]),
     "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
     "hhs_globals_set_kick_precision": (_i, [_vp, _i]),
+    "hhs_globals_set_subtree_max": (_i, [_vp, _i]),
+    "hhs_globals_subtree_max": (_i, [_vp]),
     "hhs_globals_timestep_mode": (_i, [_vp]),
     "hhs_globals_kick_precision": (_i, [_vp]),
     "hhs_globals_root_steps": (C.c_longlong, [_vp]),
@@ -423,7 +425,8 @@ class Globals:
     """nbodysystem_globals<double> + per-level buffers (hhs_globals)."""
 
     def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48,
-                 fast_timesteps: bool = False, kick_precision: int = -1):
+                 fast_timesteps: bool = False, kick_precision: int = -1,
+                 subtree_max: int | None = None):
         """`kick_precision` is a KICK_* mode for this HOLD (-1: the process-wide
         setting).  kick_precision=KICK_F64_EXACT with exact timesteps is the
         FP64-only HOLD mode, bit-exact with the FP64 CPU HOLD."""
@@ -434,6 +437,12 @@ class Globals:
             check(_lib.hhs_globals_set_timestep_mode(h, _ts_mode(fast_timesteps)))
         if kick_precision != -1:
             check(_lib.hhs_globals_set_kick_precision(h, kick_precision))
+        if subtree_max is not None:
+            check(_lib.hhs_globals_set_subtree_max(h, subtree_max))
+
+    @property
+    def subtree_max(self) -> int:
+        return _lib.hhs_globals_subtree_max(self.h)
 
     def close(self):
         if self.h:

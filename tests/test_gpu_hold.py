@@ -15,25 +15,30 @@ from conftest import plummer
 pytestmark = pytest.mark.gpu
 
 
-def _run_pair(hhs, oracle, pos, vel, m, eta, eps, t_end, dt_root):
+def _run_pair(hhs, oracle, pos, vel, m, eta, eps, t_end, dt_root, subtree_max=None):
     eps2 = eps * eps
     host = oracle.HostParticles(len(m), pos, vel, m)
     rc, ost = oracle.hold(host, t_end, dt_root, eta, eps2)
     assert rc == 0
     d = hhs.DeviceParticles.from_host(pos, vel, m)
-    g = hhs.Globals(len(m), eta, eps)
+    g = hhs.Globals(len(m), eta, eps, subtree_max=subtree_max)
     g.hold(d, t_end, dt_root)
     return host, ost, d, g
 
 
+# subtree_max 0: host recursion for every call; 64: calls with <= 64
+# particles run their subtree on the device; 4096: the whole root step does.
+@pytest.mark.parametrize("subtree_max", [0, 64, 4096])
 @pytest.mark.parametrize("n", [2, 16, 128, 512])
-def test_hold_one_root_step_matches_oracle(hhs, oracle, torch_cuda, n):
+def test_hold_one_root_step_matches_oracle(hhs, oracle, torch_cuda, n, subtree_max):
     eps, eta = 1e-2, 0.01
     pos, vel, m = plummer(hhs, n, seed=1000 + n, eps2=eps * eps)
-    host, ost, d, g = _run_pair(hhs, oracle, pos, vel, m, eta, eps, 1.0 / 32, 1.0 / 32)
+    host, ost, d, g = _run_pair(hhs, oracle, pos, vel, m, eta, eps, 1.0 / 32, 1.0 / 32,
+                                subtree_max)
     torch_cuda.cuda.synchronize()
     st = g.stats()
-    assert st["calls"] == ost.calls
+    assert st["calls"] == ost.calls and st["interactions"] == ost.interactions
+    assert st["ts_pairs"] == ost.ts_pairs and st["max_level"] == ost.max_level
     assert st["calls_per_level"] == [int(x) for x in ost.calls_per_level[:ost.max_level + 1]]
     assert g.simtime == pytest.approx(ost.simtime, rel=0, abs=0)
     p, v = d.pos.cpu().numpy(), d.vel.cpu().numpy()
@@ -143,13 +148,37 @@ def test_hold_momentum_conserved(hhs, torch_cuda):
     assert np.all(np.abs(p1) <= 1e-6 * scale)
 
 
-def test_hold_depth_guard(hhs, torch_cuda):
+@pytest.mark.parametrize("subtree_max", [0, 4096])
+def test_hold_depth_guard(hhs, torch_cuda, subtree_max):
     """Two coincident particles with eps = 0 give tau = 0: the reference
-    would recurse forever; here the level guard returns HHS_ERR_DEPTH."""
+    would recurse forever; here the level guard returns HHS_ERR_DEPTH (from
+    the host recursion or from a device subtree)."""
     pos, vel, m = plummer(hhs, 16, seed=3)
     pos[5] = pos[4]
     d = hhs.DeviceParticles.from_host(pos, vel, m)
-    g = hhs.Globals(16, 0.01, 0.0, max_level=12)
+    g = hhs.Globals(16, 0.01, 0.0, max_level=12, subtree_max=subtree_max)
     with pytest.raises(hhs.HHSError) as ei:
         g.hold_step(d, 1.0 / 64)
     assert ei.value.code == hhs.HHS_ERR_DEPTH
+
+
+@pytest.mark.parametrize("subtree_max", [32, 256])
+def test_hold_subtrees_several_root_steps(hhs, oracle, torch_cuda, subtree_max):
+    """Device subtrees inside a longer run: same recursion and simtime as
+    the FP64 CPU HOLD, trajectories within 1e-6 of the rms, and
+    momentum conserved."""
+    n, eps, eta = 400, 1e-3, 0.01
+    pos, vel, m = plummer(hhs, n, seed=4646, eps2=eps * eps)
+    host, ost, d, g = _run_pair(hhs, oracle, pos, vel, m, eta, eps, 0.125, 1.0 / 32,
+                                subtree_max)
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert g.subtree_max == subtree_max
+    assert st["calls"] == ost.calls
+    assert st["calls_per_level"] == [int(x) for x in ost.calls_per_level[:ost.max_level + 1]]
+    assert g.simtime == ost.simtime
+    p, v = d.pos.cpu().numpy(), d.vel.cpu().numpy()
+    assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
+    assert np.abs(v - host.vel).max() <= 1e-6 * np.sqrt((host.vel**2).mean())
+    mom = (m[:, None] * v).sum(0)
+    assert np.abs(mom).max() <= 1e-6 * (m[:, None] * np.abs(v)).sum(0).max()
