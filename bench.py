@@ -338,7 +338,8 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
             "speedup_vs_plain_loop": round(sps * loop_per_system, 1) if loop_t else None}
 
 
-def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
+def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5, batch_streams=0, big_workers=0,
+                small_max_n=0):
     """Served workload: Poisson arrivals of Plummer integrations with N
     log-uniform in [n_lo, n_hi], end time uniform in [0.1, 1], eta = 0.01,
     through the continuous-batching server; time-to-solution = completion -
@@ -352,8 +353,9 @@ def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
         reqs.append((p, v, m, float(rng.uniform(0.1, 1.0))))
     for rate in rates:
         gaps = np.random.default_rng(seed + int(rate * 1000)).exponential(1.0 / rate, n_req)
-        with hhs.Server(ETA, EPS, hbm_budget_bytes=64 << 30, small_max_n=2048, max_n=n_hi,
-                        batch_streams=2, big_workers=2, fast_timesteps=hhs.TIMESTEP_FP32) as srv:
+        with hhs.Server(ETA, EPS, hbm_budget_bytes=64 << 30, small_max_n=small_max_n,
+                        max_n=n_hi, batch_streams=batch_streams, big_workers=big_workers,
+                        fast_timesteps=hhs.TIMESTEP_FP32) as srv:
             tickets = []
             t0 = time.perf_counter()
             t_next = t0
@@ -367,17 +369,22 @@ def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5):
             wall = time.perf_counter() - t0
             peak = srv.stats()["peak_bytes"]
         tts = np.array([i["t_done"] - i["t_submit"] for i in infos])
+        wait = np.array([i["t_start"] - i["t_submit"] for i in infos])
+        run = np.array([i["t_done"] - i["t_start"] for i in infos])
         de = np.array([abs((i["result"]["e1"] - i["result"]["e0"]) / i["result"]["e0"])
                        for i in infos])
         out.append({"rate_per_s": rate, "requests": n_req,
                     "ok": int(sum(i["state"] == 3 for i in infos)),
                     "p50_s": round(float(np.median(tts)), 4),
                     "p99_s": round(float(np.quantile(tts, 0.99)), 4),
+                    "queue_wait_mean_s": round(float(wait.mean()), 4),
+                    "run_mean_s": round(float(run.mean()), 4),
                     "throughput_req_per_s": round(n_req / wall, 3),
                     "wall_s": round(wall, 3), "peak_admitted_bytes": int(peak),
                     "frac_dE_le_1e-6": round(float(np.mean(de <= 1e-6)), 4)})
     return {"n_range": [n_lo, n_hi], "t_end_range": [0.1, 1.0], "dt_root": 1.0 / 16,
-            "eta": ETA, "eps": EPS, "timesteps": "fp32", "sweep": out}
+            "eta": ETA, "eps": EPS, "timesteps": "fp32", "small_max_n": small_max_n or 512,
+            "batch_streams": batch_streams or 2, "big_workers": big_workers or 4, "sweep": out}
 
 
 def main():

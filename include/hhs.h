@@ -403,6 +403,11 @@ typedef struct hhs_ensemble_desc {
     int wave_team_max_n;   /* 0 selects 256                                   */
     hhs_system_result *results; /* device, per system                        */
     int timestep_mode;     /* HHS_TIMESTEP_EXACT, _FAST or _FP32               */
+    int *done;             /* optional, per system: set to 1 (system-scope
+                              release) once that system's pos, vel and result
+                              are written, while the launch still runs; with
+                              pos/vel/results/done in host-mapped memory a
+                              host thread can finish systems one by one      */
 } hhs_ensemble_desc;
 
 size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles);
@@ -425,11 +430,11 @@ typedef struct hhs_server hhs_server;
 typedef struct hhs_server_config {
     double eta, eps;
     size_t hbm_budget_bytes;
-    int small_max_n;    /* 0 selects 2048                                   */
+    int small_max_n;    /* 0 selects 512                                    */
     int max_n;          /* largest accepted system, 0 selects 65536         */
     int max_batch;      /* systems per ensemble batch, 0 selects 4096       */
     int batch_streams;  /* ensemble batches in flight, 0 selects 2          */
-    int big_workers;    /* concurrent large systems, 0 selects 2            */
+    int big_workers;    /* concurrent large systems, 0 selects 4            */
     int max_level;      /* 0 selects 40                                     */
     int timestep_mode;  /* HHS_TIMESTEP_EXACT, _FAST or _FP32                */
 } hhs_server_config;

@@ -68,6 +68,7 @@ struct EnsArgs {
     int n_small;
     unsigned int *counters;
     hhs_system_result *results;
+    int *done;
     int ts_mode;
 };
 
@@ -487,6 +488,10 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
             }
         }
     }
+    if (a.done) {
+        __threadfence_system();  // this thread's pos/vel stores, before the flag
+        tm.sync();
+    }
     if (tm.t == 0) {
         hhs_system_result r;
         r.e0 = e0;
@@ -498,6 +503,8 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
         r.max_level = st.max_level;
         r.status = status;
         a.results[s] = r;
+        if (a.done)
+            __hip_atomic_store(&a.done[s], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     tm.sync();
 }
@@ -744,6 +751,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     a.n_small = static_cast<int>(small.size());
     a.counters = counters;
     a.results = d->results;
+    a.done = d->done;
     HHS_REQUIRE(d->timestep_mode == HHS_TIMESTEP_EXACT || d->timestep_mode == HHS_TIMESTEP_FAST ||
                     d->timestep_mode == HHS_TIMESTEP_FP32,
                 "ensemble: bad timestep_mode %d", d->timestep_mode);

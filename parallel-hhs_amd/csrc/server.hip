@@ -7,7 +7,9 @@
 // device as slots and memory free up:
 //  * small systems (n <= small_max_n) are gathered into a batch and run by
 //    one persistent ensemble launch (hhs_ensemble_run_f64) on one of
-//    `batch_streams` streams -- a new batch can start while older ones run;
+//    `batch_streams` streams -- a new batch can start while older ones run,
+//    and each request of a batch is handed back as soon as its own system
+//    is done (per-system flags in host-mapped memory);
 //  * large systems run the per-system HOLD driver (hhs_hold_f64) on one of
 //    `big_workers` worker threads, each with its own stream and a reusable
 //    hhs_globals sized for max_n.
@@ -48,13 +50,16 @@ struct Request {
 
 struct Batch {
     std::vector<Request *> reqs;
+    std::vector<size_t> first;    // particle offset of each request
+    std::vector<char> finished;   // request already handed back
     int slot = -1;
     hipEvent_t done = nullptr;
-    void *dev = nullptr;   // one stream-ordered allocation
-    void *host = nullptr;  // pinned staging
+    void *dev = nullptr;   // one stream-ordered allocation (inputs, workspace)
+    void *host = nullptr;  // host-mapped, coherent: pos/vel in and out, results, flags
     size_t n_part = 0;
     double *h_pos = nullptr, *h_vel = nullptr;
     hhs_system_result *h_res = nullptr;
+    int *h_done = nullptr;
 };
 
 size_t small_footprint(int n) {
@@ -104,7 +109,12 @@ void fail_request(hhs_server *s, Request *r, int status) {
     s->used -= r->footprint;
 }
 
-// Launch one ensemble batch (called without the lock held).
+// Launch one ensemble batch (called without the lock held).  Positions,
+// velocities, results and per-system done flags live in host-mapped
+// coherent memory: the kernel reads the initial state from it, writes each
+// system's final state and result there and then raises that system's flag,
+// so the scheduler can hand back every request as soon as its own system
+// is done instead of when the whole batch is.
 int launch_batch(hhs_server *s, Batch *b) {
     const int k = static_cast<int>(b->reqs.size());
     hipStream_t st = s->batch_streams[b->slot];
@@ -112,20 +122,22 @@ int launch_batch(hhs_server *s, Batch *b) {
     for (int q = 0; q < k; ++q) off[q + 1] = off[q] + b->reqs[q]->n;
     const size_t np = static_cast<size_t>(off[k]);
     b->n_part = np;
+    b->first.assign(off.begin(), off.end() - 1);
+    b->finished.assign(k, 0);
     const size_t wsb = hhs_ensemble_workspace_bytes(k, static_cast<long long>(np));
     auto al = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
-    // device: offsets, pos, vel, m, t_end, dt_root, results, workspace
-    const size_t o_off = 0, o_pos = o_off + al((k + 1) * 4), o_vel = o_pos + al(np * 24 + 8),
-                 o_m = o_vel + al(np * 24 + 8), o_te = o_m + al(np * 8 + 8),
-                 o_dt = o_te + al(k * 8), o_res = o_dt + al(k * 8),
-                 o_ws = o_res + al(k * sizeof(hhs_system_result)), dev_bytes = o_ws + al(wsb);
-    // host staging mirrors the first six blocks, then results
-    const size_t h_bytes = o_res + al(k * sizeof(hhs_system_result));
+    // device: offsets, m, t_end, dt_root, workspace
+    const size_t o_off = 0, o_m = o_off + al((k + 1) * 4), o_te = o_m + al(np * 8 + 8),
+                 o_dt = o_te + al(k * 8), o_ws = o_dt + al(k * 8), dev_bytes = o_ws + al(wsb);
+    // host-mapped: staging of the device inputs, then pos, vel, results, flags
+    const size_t h_pos = o_ws, h_vel = h_pos + al(np * 24 + 8), h_res = h_vel + al(np * 24 + 8),
+                 h_done = h_res + al(k * sizeof(hhs_system_result)),
+                 h_bytes = h_done + al(k * sizeof(int));
     HHS_HIP(hipMallocAsync(&b->dev, dev_bytes, st));
-    HHS_HIP(hipHostMalloc(&b->host, h_bytes, hipHostMallocDefault));
+    HHS_HIP(hipHostMalloc(&b->host, h_bytes, hipHostMallocMapped | hipHostMallocCoherent));
     char *h = static_cast<char *>(b->host);
     std::memcpy(h + o_off, off.data(), (k + 1) * 4);
-    double *hp = reinterpret_cast<double *>(h + o_pos), *hv = reinterpret_cast<double *>(h + o_vel),
+    double *hp = reinterpret_cast<double *>(h + h_pos), *hv = reinterpret_cast<double *>(h + h_vel),
            *hm = reinterpret_cast<double *>(h + o_m), *hte = reinterpret_cast<double *>(h + o_te),
            *hdt = reinterpret_cast<double *>(h + o_dt);
     for (int q = 0; q < k; ++q) {
@@ -138,14 +150,19 @@ int launch_batch(hhs_server *s, Batch *b) {
     }
     b->h_pos = hp;
     b->h_vel = hv;
-    b->h_res = reinterpret_cast<hhs_system_result *>(h + o_res);
+    b->h_res = reinterpret_cast<hhs_system_result *>(h + h_res);
+    b->h_done = reinterpret_cast<int *>(h + h_done);
+    std::memset(b->h_done, 0, k * sizeof(int));
+    void *hdev = nullptr;
+    HHS_HIP(hipHostGetDevicePointer(&hdev, b->host, 0));
+    char *hd = static_cast<char *>(hdev);
     char *d = static_cast<char *>(b->dev);
-    HHS_HIP(hipMemcpyAsync(d, h, o_res, hipMemcpyHostToDevice, st));
+    HHS_HIP(hipMemcpyAsync(d, h, o_ws, hipMemcpyHostToDevice, st));
     hhs_ensemble_desc desc{};
     desc.n_systems = k;
     desc.offsets = reinterpret_cast<const int *>(d + o_off);
-    desc.pos = reinterpret_cast<double *>(d + o_pos);
-    desc.vel = reinterpret_cast<double *>(d + o_vel);
+    desc.pos = reinterpret_cast<double *>(hd + h_pos);
+    desc.vel = reinterpret_cast<double *>(hd + h_vel);
     desc.m = reinterpret_cast<const double *>(d + o_m);
     desc.t_end = reinterpret_cast<const double *>(d + o_te);
     desc.dt_root = reinterpret_cast<const double *>(d + o_dt);
@@ -154,35 +171,51 @@ int launch_batch(hhs_server *s, Batch *b) {
     desc.max_level = s->cfg.max_level;
     desc.max_calls = 0;
     desc.wave_team_max_n = 0;
-    desc.results = reinterpret_cast<hhs_system_result *>(d + o_res);
+    desc.results = reinterpret_cast<hhs_system_result *>(hd + h_res);
     desc.timestep_mode = s->cfg.timestep_mode;
+    desc.done = reinterpret_cast<int *>(hd + h_done);
     int rc = hhs_ensemble_run_f64(&desc, off.data(), d + o_ws, wsb, st);
     if (rc) return rc;
-    HHS_HIP(hipMemcpyAsync(h + o_pos, d + o_pos, o_te - o_pos, hipMemcpyDeviceToHost, st));
-    HHS_HIP(hipMemcpyAsync(h + o_res, d + o_res, k * sizeof(hhs_system_result),
-                           hipMemcpyDeviceToHost, st));
     HHS_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     HHS_HIP(hipEventRecord(b->done, st));
     return HHS_OK;
 }
 
-// Copy a finished batch back into its requests (lock held).
-void finish_batch(hhs_server *s, Batch *b, int rc) {
-    size_t o = 0;
+// Hand back request q of a batch whose flag is up (lock held).
+void finish_request(hhs_server *s, Batch *b, size_t q, double t) {
+    Request *r = b->reqs[q];
+    std::memcpy(r->pos.data(), b->h_pos + 3 * b->first[q], r->n * 24);
+    std::memcpy(r->vel.data(), b->h_vel + 3 * b->first[q], r->n * 24);
+    r->info.result = b->h_res[q];
+    r->info.state = b->h_res[q].status == HHS_OK ? kDone : kFailed;
+    r->info.t_done = t;
+    s->used -= r->footprint;
+    b->finished[q] = 1;
+}
+
+// Requests of a running batch whose systems are done (lock held).
+bool poll_batch(hhs_server *s, Batch *b) {
+    bool any = false;
     const double t = s->now();
     for (size_t q = 0; q < b->reqs.size(); ++q) {
-        Request *r = b->reqs[q];
-        if (rc == HHS_OK) {
-            std::memcpy(r->pos.data(), b->h_pos + 3 * o, r->n * 24);
-            std::memcpy(r->vel.data(), b->h_vel + 3 * o, r->n * 24);
-            r->info.result = b->h_res[q];
-            r->info.state = b->h_res[q].status == HHS_OK ? kDone : kFailed;
-            r->info.t_done = t;
-            s->used -= r->footprint;
-        } else {
-            fail_request(s, r, rc);
+        if (b->finished[q]) continue;
+        if (__atomic_load_n(&b->h_done[q], __ATOMIC_ACQUIRE) == 1) {
+            finish_request(s, b, q, t);
+            any = true;
         }
-        o += r->n;
+    }
+    return any;
+}
+
+// Retire a batch after its launch has completed (lock held).
+void finish_batch(hhs_server *s, Batch *b, int rc) {
+    const double t = s->now();
+    for (size_t q = 0; q < b->reqs.size(); ++q) {
+        if (q < b->finished.size() && b->finished[q]) continue;
+        if (rc == HHS_OK && b->h_done && __atomic_load_n(&b->h_done[q], __ATOMIC_ACQUIRE) == 1)
+            finish_request(s, b, q, t);
+        else
+            fail_request(s, b->reqs[q], rc == HHS_OK ? HHS_ERR_HIP : rc);
     }
     hipStream_t st = s->batch_streams[b->slot];
     if (b->dev) (void)hipFreeAsync(b->dev, st);
@@ -234,9 +267,11 @@ void scheduler_loop(hhs_server *s) {
             else
                 s->inflight.push_back(b);
         }
-        // 3. completions
+        // 3. completions: single systems as their flags come up, then whole
+        //    launches
         for (size_t q = 0; q < s->inflight.size();) {
             Batch *b = s->inflight[q];
+            if (poll_batch(s, b)) s->cv.notify_all();
             const hipError_t e = hipEventQuery(b->done);
             if (e == hipErrorNotReady) {
                 ++q;
@@ -347,10 +382,15 @@ int hhs_server_create(hhs_server **out, const hhs_server_config *cfg) {
     auto *s = new (std::nothrow) hhs_server();
     if (!s) return HHS_ERR_ALLOC;
     s->cfg = *cfg;
-    if (s->cfg.small_max_n <= 0) s->cfg.small_max_n = 2048;
+    // Defaults from the MI355X sweep of a Poisson stream (N log-uniform in
+    // [64, 4096], t_end in [0.1, 1], lambda = 4 and 16 per s): N <= 512 in
+    // ensemble batches on 2 streams and 4 per-system workers gave the lowest
+    // p99 (1.9 s / 3.2 s) -- larger systems finish sooner on the whole GPU
+    // (with device subtrees) than on one workgroup team of a batch.
+    if (s->cfg.small_max_n <= 0) s->cfg.small_max_n = 512;
     if (s->cfg.max_n <= 0) s->cfg.max_n = 65536;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 4096;
-    if (s->cfg.big_workers <= 0) s->cfg.big_workers = 2;
+    if (s->cfg.big_workers <= 0) s->cfg.big_workers = 4;
     if (s->cfg.batch_streams <= 0) s->cfg.batch_streams = 2;
     if (s->cfg.max_level <= 0) s->cfg.max_level = 40;
     s->t0 = std::chrono::steady_clock::now();

@@ -80,7 +80,7 @@ class EnsembleDesc(C.Structure):
                 ("dt_root", C.c_void_p), ("eta", C.c_double), ("eps", C.c_double),
                 ("max_level", C.c_int), ("max_calls", C.c_longlong),
                 ("wave_team_max_n", C.c_int), ("results", C.c_void_p),
-                ("timestep_mode", C.c_int)]
+                ("timestep_mode", C.c_int), ("done", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):

@@ -64,7 +64,7 @@ def test_server_budget_admission(hhs, torch_cuda):
     reqs = _reqs(hhs, [1000] * 6, 5)
     per = 1000 * (56 + 136) + 512
     budget = 2 * per + 100  # two small requests fit at a time
-    with hhs.Server(ETA, EPS, hbm_budget_bytes=budget) as srv:
+    with hhs.Server(ETA, EPS, hbm_budget_bytes=budget, small_max_n=2048) as srv:
         ts = [srv.submit(p, v, m, te, dtr) for (p, v, m, te, dtr) in reqs]
         infos = [srv.wait(t) for t in ts]
         st = srv.stats()
