@@ -400,7 +400,7 @@ typedef struct hhs_ensemble_desc {
     double eta, eps;
     int max_level;         /* <= 47; 0 selects 47                             */
     long long max_calls;   /* per-system cap on hold_step calls, 0 = none     */
-    int wave_team_max_n;   /* 0 selects 256                                   */
+    int wave_team_max_n;   /* 0 selects 512                                   */
     hhs_system_result *results; /* device, per system                        */
     int timestep_mode;     /* HHS_TIMESTEP_EXACT, _FAST or _FP32               */
     int *done;             /* optional, per system: set to 1 (system-scope

@@ -698,7 +698,12 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     const EnsLayout L = layout(nsys, np);
     HHS_REQUIRE(ws && ws_bytes >= L.total, "ensemble: workspace of %zu bytes, need %zu", ws_bytes,
                 L.total);
-    const int wave_max = d->wave_team_max_n > 0 ? d->wave_team_max_n : 256;
+    // One-wave teams up to 512 particles: the MI355X sweep of the
+    // 4096-system workload gave 6.4-6.7 s against 7.8-9.4 s with 256.
+    // Larger workgroup teams (512 or 1024 threads per big system) were
+    // slower (7.1 s, 10.7 s): the recursion's per-call synchronisation, not
+    // the lane count, bounds a big system.
+    const int wave_max = d->wave_team_max_n > 0 ? d->wave_team_max_n : 512;
 
     // Largest-first queues: whole-workgroup systems and one-wave systems.
     std::vector<int> big, small;
