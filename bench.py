@@ -201,27 +201,49 @@ def hold_bench(hhs, torch, pos, vel, m, dt, label, **modes):
 
 
 def timesteps_bench(hhs, torch, d, stream, reps=3):
-    """The timestep kernel over all N = 131072 particles in its three modes,
-    priced at 49 flops per ordered pair (the reference formula counting
-    each +, -, *, /, sqrt and min as one)."""
+    """The timestep kernels over all N = 131072 particles: the one-sided
+    kernel in its three modes and the symmetric kernel (each unordered pair
+    once) in EXACT and FP32, priced at 49 flops per evaluated pair (the
+    reference formula counting each +, -, *, /, sqrt and min as one)."""
     out = []
     pairs = float(N) * (N - 1)
-    for name, mode, peak in (("exact", 0, FP64_PEAK_TFLOPS), ("fast", 1, FP64_PEAK_TFLOPS),
-                             ("fp32", 2, FP32_PEAK_TFLOPS)):
-        hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=mode)
+    import ctypes as C
+    nbytes = int(hhs.raw().hhs_timesteps_sym_workspace_bytes(N))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    s = d.struct()
+
+    def sym(mode):
+        hhs.check(hhs.raw().hhs_timesteps_sym_f64(C.byref(s), N, ETA, EPS * EPS, mode,
+                                                  C.c_void_p(ws.data_ptr()), nbytes,
+                                                  C.c_void_p(stream.cuda_stream)))
+
+    for name, mode, peak, symmetric in (
+            ("exact", 0, FP64_PEAK_TFLOPS, False), ("fast", 1, FP64_PEAK_TFLOPS, False),
+            ("fp32", 2, FP32_PEAK_TFLOPS, False), ("exact_symmetric", 0, FP64_PEAK_TFLOPS, True),
+            ("fp32_symmetric", 2, FP32_PEAK_TFLOPS, True)):
+        def go():
+            if symmetric:
+                sym(mode)
+            else:
+                hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=mode)
+        go()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(reps):
-            hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=mode)
+            go()
         ev1.record(stream)
         torch.cuda.synchronize()
         ms = ev0.elapsed_time(ev1) / reps
-        tf = TS_FLOPS_PER_PAIR * pairs / (ms * 1e-3) / 1e12
+        # flops at the evaluated pairs: ordered pairs for the one-sided kernel,
+        # unordered pairs (half) for the symmetric one
+        evaluated = pairs / 2 if symmetric else pairs
+        tf = TS_FLOPS_PER_PAIR * evaluated / (ms * 1e-3) / 1e12
         out.append({"mode": name, "ms": round(ms, 3),
                     "Gpairs_per_s": round(pairs / (ms * 1e-3) / 1e9, 1),
+                    "evaluated_pairs": "unordered" if symmetric else "ordered",
                     "TFLOP_per_s": round(tf, 2), "peak": peak, "frac": round(tf / peak, 4)})
-    return {"n": N, "pairs": "ordered, i != j", "flops_per_pair": TS_FLOPS_PER_PAIR,
-            "modes": out}
+    return {"n": N, "pairs": "Gpairs_per_s counts ordered pairs (i != j) for every mode",
+            "flops_per_pair": TS_FLOPS_PER_PAIR, "modes": out}
 
 
 def stream_bench(hhs, torch, stream, sizes=(N, 1 << 20, 1 << 22, 1 << 24), reps=10):

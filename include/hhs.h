@@ -167,6 +167,16 @@ int hhs_kick_get_config(int n_sink, int n_src, int *sinks_per_lane, int *waves_p
  * FP64 CPU evaluation in that order. */
 int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2,
                       void *stream);
+/* Symmetric evaluation of the same minimum: tau_ij = tau_ji bit for bit, so
+ * each unordered pair is evaluated once and feeds both particles (blocks of
+ * 1024 particles, partial minima in `ws`, then a min-reduction; no atomics).
+ * HHS_TIMESTEP_EXACT stays bit-exact with hhs_timesteps_f64; FP32 is within
+ * 1e-6; HHS_TIMESTEP_FAST uses the one-sided kernel.  ws must hold
+ * hhs_timesteps_sym_workspace_bytes(n) bytes (ceil(n/1024) * n doubles).
+ * No reference counterpart (the reference evaluates every ordered pair). */
+size_t hhs_timesteps_sym_workspace_bytes(int n);
+int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
+                          void *ws, size_t ws_bytes, void *stream);
 int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,
                       void *stream);
 

@@ -88,6 +88,17 @@ int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta
                      hipStream_t s);
 int launch_timesteps_mode(int mode, const double *pos, const double *vel, const double *m,
                           double *ts, int n, double eta, double eps2, hipStream_t s);
+// Symmetric evaluation (each unordered pair once) for the EXACT and FP32
+// modes (FAST falls back to the one-sided kernel); ws holds
+// timesteps_sym_ws_bytes(n) bytes of partial minima.  HOLD uses it from
+// kSymMinN particles up, where it beats the one-sided kernel (MI355X, ms:
+// n = 32768 EXACT 6.02 vs 6.21, FP32 1.81 vs 1.63; n = 65536 17.1 vs 22.4,
+// 4.80 vs 5.37; n = 131072 56.2 vs 92.4, 15.1 vs 21.7).
+constexpr int kSymMinN = 65536;
+size_t timesteps_sym_ws_bytes(int n);
+int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
+                         double *ts, int n, double eta, double eps2, void *ws, size_t ws_bytes,
+                         hipStream_t s);
 
 // ----------------------------------------------------------- scan / split
 // Workspace layout: u32 header[16] (header[2] = raw fast count of the last

@@ -50,6 +50,8 @@ struct hhs_globals {
     // Calls with at most subtree_max particles run their whole subtree in one
     // device launch (0 = never).
     int subtree_max = 0;
+    void *ts_ws = nullptr;  // partial minima of the symmetric timestep kernel
+    size_t ts_ws_bytes = 0;
     void *sub_ws = nullptr;
     size_t sub_ws_bytes = 0;
     hhs::SubtreeStats *sub_dev = nullptr;
@@ -175,8 +177,13 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     int rc;
     // src/hold.cuh:74-76
     if (calc) {
-        rc = launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m,
-                                   total.timestep, n, g->eta, g->eps2, st);
+        if (g->ts_ws && n >= kSymMinN)
+            rc = launch_timesteps_sym(g->timestep_mode, total.pos, total.vel, total.m,
+                                      total.timestep, n, g->eta, g->eps2, g->ts_ws,
+                                      g->ts_ws_bytes, st);
+        else
+            rc = launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m,
+                                       total.timestep, n, g->eta, g->eps2, st);
         if (rc) return rc;
         S.ts_pairs += static_cast<long long>(n) * (n - 1);
     }
@@ -277,6 +284,17 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
         hhs_globals_destroy(g);
         return rc;
     }
+    // Symmetric timesteps from kSymMinN particles, when the partial minima
+    // (ceil(n/1024) * n doubles, 134 MB at n = 131072) stay within 2 GiB.
+    const size_t tsb = timesteps_sym_ws_bytes(n_total);
+    if (n_total >= kSymMinN && tsb <= (size_t{2} << 30)) {
+        if (hipMalloc(&g->ts_ws, tsb) != hipSuccess) {
+            int rc = hip_status(hipErrorOutOfMemory, "globals_create", __FILE__, __LINE__);
+            hhs_globals_destroy(g);
+            return rc;
+        }
+        g->ts_ws_bytes = tsb;
+    }
     *out = g;
     return HHS_OK;
 }
@@ -291,6 +309,7 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->part_dev) (void)hipFree(g->part_dev);
     if (g->part_host) (void)hipHostFree(g->part_host);
     if (g->sub_ws) (void)hipFree(g->sub_ws);
+    if (g->ts_ws) (void)hipFree(g->ts_ws);
     if (g->sub_dev) (void)hipFree(g->sub_dev);
     if (g->sub_host) (void)hipHostFree(g->sub_host);
     delete g;

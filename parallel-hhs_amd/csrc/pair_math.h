@@ -142,6 +142,32 @@ __device__ __forceinline__ void pair_timestep_f32(double xi, double yi, double z
     fold_fraction32(tfb, __builtin_fmaf(-0.5f, d, 1.0f), other, bnum, bden);
 }
 
+// The pair's own timestep min(tau_ff, tau_fb) (start 1e30, NaN candidates
+// ignored), from the same arithmetic as pair_timestep.  Symmetric in (i, j)
+// bit for bit: the formula sees the pair only through r^2, v^2, v.r (signs
+// of both differences flip together, exactly) and m_i + m_j (commutative).
+template <typename T>
+__device__ __forceinline__ T pair_tau(T xi, T yi, T zi, T vxi, T vyi, T vzi, T mi, T xj, T yj,
+                                      T zj, T vxj, T vyj, T vzj, T mj, T eta, T c_ff, T eps2) {
+    T best = TsConst<T>::huge;
+    pair_timestep<T>(xi, yi, zi, vxi, vyi, vzi, mi, xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2,
+                     true, best);
+    return best;
+}
+
+// FP32 form: the fraction of pair_timestep_f32 turned into a value with the
+// hardware reciprocal (~1 ulp).
+__device__ __forceinline__ float pair_tau_f32(double xi, double yi, double zi, double vxi,
+                                              double vyi, double vzi, float mi, double xj,
+                                              double yj, double zj, double vxj, double vyj,
+                                              double vzj, float mj, float eta, float c_ff,
+                                              float eps2) {
+    float bnum = TsConst<float>::huge, bden = 1.0f;
+    pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mi, xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff,
+                      eps2, true, bnum, bden);
+    return bnum * __builtin_amdgcn_rcpf(bden);
+}
+
 // Knuth two-sum of hi + p; the rounding error is carried into lo.
 __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float p) {
     const float s = hi + p;

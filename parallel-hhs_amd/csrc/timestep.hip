@@ -156,6 +156,176 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
     return HHS_ERR_INVALID;
 }
 
+namespace {
+// ---------------------------------------------------- symmetric evaluation
+// tau_ij = tau_ji bit for bit (pair_tau), so each unordered pair is
+// evaluated once and feeds the minima of both particles.  Particles are cut
+// into blocks of kSymBlock; workgroup (I, J), I <= J, takes sink block I
+// (kSymSpl sinks per lane, 4 waves) against source block J (one wave-uniform
+// source at a time, scalar loads).  It keeps the row minima of its sinks in
+// registers and reduces each source's column minimum across the wave
+// (shuffles) and the 4 waves (LDS).  Partial minima go to P[K][i], the
+// minimum of particle i over block K, written exactly once:
+//   (I, J), I < J: P[J][i] for i in I (rows), P[I][j] for j in J (columns);
+//   (I, I): P[I][i], all j != i of the block (rows only).
+// A second kernel takes tau_i = min_K P[K][i].  No atomics; the minimum of
+// a set does not depend on the order, so the exact mode stays bit-exact.
+constexpr int kSymSpl = 4;                    // sinks per lane
+constexpr int kSymBlock = 64 * 4 * kSymSpl;   // 1024 particles per block
+
+template <int MODE> struct SymV { using type = double; };
+template <> struct SymV<HHS_TIMESTEP_FP32> { using type = float; };
+
+template <typename V> __device__ __forceinline__ V vmin(V a, V b) { return b < a ? b : a; }
+
+template <typename V> __device__ __forceinline__ V wave_min(V x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = vmin(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+template <int MODE>
+__device__ __forceinline__ typename SymV<MODE>::type sym_pair(
+    double xi, double yi, double zi, double vxi, double vyi, double vzi, double mi, float mif,
+    double xj, double yj, double zj, double vxj, double vyj, double vzj, double mj, double eta,
+    double c_ff, double eps2) {
+    if constexpr (MODE == HHS_TIMESTEP_FP32)
+        return pair_tau_f32(xi, yi, zi, vxi, vyi, vzi, mif, xj, yj, zj, vxj, vyj, vzj,
+                            static_cast<float>(mj), static_cast<float>(eta),
+                            static_cast<float>(c_ff), static_cast<float>(eps2));
+    else
+        return pair_tau<double>(xi, yi, zi, vxi, vyi, vzi, mi, xj, yj, zj, vxj, vyj, vzj, mj, eta,
+                                c_ff, eps2);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void timesteps_sym_kernel(
+    const double *__restrict__ pos, const double *__restrict__ vel, const double *__restrict__ m,
+    int n, double eta, double eps2, typename SymV<MODE>::type *__restrict__ part) {
+    using V = typename SymV<MODE>::type;
+    const int I = blockIdx.x, J = blockIdx.y;
+    if (I > J) return;
+    __shared__ V colmin[4][kSymBlock];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double c_ff = eta / TsConst<double>::sqrt2;
+    const V huge = static_cast<V>(TsConst<double>::huge);
+    double xi[kSymSpl], yi[kSymSpl], zi[kSymSpl], vxi[kSymSpl], vyi[kSymSpl], vzi[kSymSpl],
+        mi[kSymSpl];
+    float mif[kSymSpl];
+    int ii[kSymSpl];
+    V row[kSymSpl];
+#pragma unroll
+    for (int s = 0; s < kSymSpl; ++s) {
+        ii[s] = I * kSymBlock + s * 256 + wave * 64 + lane;
+        const int ic = ii[s] < n ? ii[s] : n - 1;
+        xi[s] = pos[3 * ic];
+        yi[s] = pos[3 * ic + 1];
+        zi[s] = pos[3 * ic + 2];
+        vxi[s] = vel[3 * ic];
+        vyi[s] = vel[3 * ic + 1];
+        vzi[s] = vel[3 * ic + 2];
+        mi[s] = m[ic];
+        mif[s] = static_cast<float>(mi[s]);
+        row[s] = huge;
+    }
+    const int j0 = J * kSymBlock;
+    const int j1 = min(j0 + kSymBlock, n);
+    if (I == J) {
+        for (int j = j0; j < j1; ++j) {
+            const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
+            const double vxj = vel[3 * j], vyj = vel[3 * j + 1], vzj = vel[3 * j + 2];
+            const double mj = m[j];
+#pragma unroll
+            for (int s = 0; s < kSymSpl; ++s) {
+                const V v = sym_pair<MODE>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
+                                           mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
+                row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
+            }
+        }
+    } else {
+        for (int j = j0; j < j1; ++j) {
+            const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
+            const double vxj = vel[3 * j], vyj = vel[3 * j + 1], vzj = vel[3 * j + 2];
+            const double mj = m[j];
+            V c = huge;
+#pragma unroll
+            for (int s = 0; s < kSymSpl; ++s) {
+                const V v = sym_pair<MODE>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
+                                           mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
+                row[s] = vmin(row[s], v);
+                c = (ii[s] < n) ? vmin(c, v) : c;
+            }
+            c = wave_min(c);
+            if (lane == 0) colmin[wave][j - j0] = c;
+        }
+        __syncthreads();
+        for (int jj = threadIdx.x; jj < j1 - j0; jj += 256) {
+            const V c = vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
+            part[static_cast<size_t>(I) * n + j0 + jj] = c;  // P[I][j]
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kSymSpl; ++s)
+        if (ii[s] < n) part[static_cast<size_t>(J) * n + ii[s]] = row[s];  // P[J][i]
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void timesteps_sym_reduce_kernel(const V *__restrict__ part,
+                                                                   int nb, int n,
+                                                                   double *__restrict__ ts) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    V best = part[i];
+    for (int k = 1; k < nb; ++k) best = vmin(best, part[static_cast<size_t>(k) * n + i]);
+    ts[i] = static_cast<double>(best);
+}
+
+}  // namespace
+
+size_t timesteps_sym_ws_bytes(int n) {
+    if (n <= 0) return 0;
+    const size_t nb = (static_cast<size_t>(n) + kSymBlock - 1) / kSymBlock;
+    return nb * static_cast<size_t>(n) * sizeof(double);
+}
+
+int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
+                         double *ts, int n, double eta, double eps2, void *ws, size_t ws_bytes,
+                         hipStream_t s) {
+    HHS_REQUIRE(n >= 0, "timesteps: negative n");
+    if (n == 0) return HHS_OK;
+    if (mode == HHS_TIMESTEP_FAST)
+        return launch_timesteps_mode(mode, pos, vel, m, ts, n, eta, eps2, s);
+    HHS_REQUIRE(mode == HHS_TIMESTEP_EXACT || mode == HHS_TIMESTEP_FP32,
+                "timesteps: unknown mode %d", mode);
+    HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
+    HHS_REQUIRE(ws && ws_bytes >= timesteps_sym_ws_bytes(n),
+                "timesteps: symmetric workspace of %zu bytes, need %zu", ws_bytes,
+                timesteps_sym_ws_bytes(n));
+    const int nb = (n + kSymBlock - 1) / kSymBlock;
+    const dim3 grid(nb, nb), red((n + 255) / 256);
+    if (mode == HHS_TIMESTEP_FP32) {
+        float *part = static_cast<float *>(ws);
+        hipLaunchKernelGGL(timesteps_sym_kernel<HHS_TIMESTEP_FP32>, grid, dim3(256), 0, s, pos, vel,
+                           m, n, eta, eps2, part);
+        HHS_LAUNCH_CHECK("timesteps_sym_kernel");
+        hipLaunchKernelGGL(timesteps_sym_reduce_kernel<float>, red, dim3(256), 0, s, part, nb, n,
+                           ts);
+    } else {
+        double *part = static_cast<double *>(ws);
+        hipLaunchKernelGGL(timesteps_sym_kernel<HHS_TIMESTEP_EXACT>, grid, dim3(256), 0, s, pos,
+                           vel, m, n, eta, eps2, part);
+        HHS_LAUNCH_CHECK("timesteps_sym_kernel");
+        hipLaunchKernelGGL(timesteps_sym_reduce_kernel<double>, red, dim3(256), 0, s, part, nb, n,
+                           ts);
+    }
+    HHS_LAUNCH_CHECK("timesteps_sym_reduce_kernel");
+    return HHS_OK;
+}
+
+namespace {
+}  // namespace
+
 template int launch_timesteps<double>(const double *, const double *, const double *, double *,
                                       int, double, double, hipStream_t);
 template int launch_timesteps<float>(const float *, const float *, const float *, float *, int,
@@ -189,6 +359,16 @@ int hhs_timesteps_mode_f64(const hhs_particles_f64 *sys, int n, double eta, doub
     HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
     return launch_timesteps_mode(mode, sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
                                  as_stream(stream));
+}
+
+size_t hhs_timesteps_sym_workspace_bytes(int n) { return timesteps_sym_ws_bytes(n); }
+
+int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
+                          void *ws, size_t ws_bytes, void *stream) {
+    HHS_REQUIRE(sys, "timesteps: null system");
+    HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
+    return launch_timesteps_sym(mode, sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2, ws,
+                                ws_bytes, as_stream(stream));
 }
 
 int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,

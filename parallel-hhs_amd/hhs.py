@@ -154,6 +154,8 @@ _SIGS = {
     "hhs_timesteps_mode_f64": (_i, [_P64, _i, _d, _d, _i, _vp]),
     "hhs_globals_set_timestep_mode": (_i, [_vp, _i]),
     "hhs_globals_set_kick_precision": (_i, [_vp, _i]),
+    "hhs_timesteps_sym_workspace_bytes": (_sz, [_i]),
+    "hhs_timesteps_sym_f64": (_i, [_P64, _i, _d, _d, _i, _vp, _sz, _vp]),
     "hhs_globals_set_subtree_max": (_i, [_vp, _i]),
     "hhs_globals_subtree_max": (_i, [_vp]),
     "hhs_globals_timestep_mode": (_i, [_vp]),
@@ -351,6 +353,19 @@ def _ts_mode(fast):
     if fast is False or fast is None:
         return TIMESTEP_EXACT
     return int(fast)
+
+
+def timesteps_sym(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, mode=0, stream=None):
+    """hhs_timesteps_sym_f64: each unordered pair evaluated once (EXACT stays
+    bit-exact; FP32 within 1e-6).  Allocates the partial-minima workspace."""
+    import torch
+    n = sys.n if n is None else n
+    nbytes = int(_lib.hhs_timesteps_sym_workspace_bytes(n))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device="cuda")
+    s = sys.struct()
+    check(_lib.hhs_timesteps_sym_f64(C.byref(s), n, eta, eps2, _ts_mode(mode), _ptr(ws), nbytes,
+                                     _stream(stream)))
+    return ws
 
 
 def timesteps(sys: DeviceParticles, n=None, eta=0.01, eps2=0.0, stream=None, fast=False):

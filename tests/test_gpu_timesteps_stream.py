@@ -144,3 +144,37 @@ def test_energy_matches_oracle(hhs, oracle, torch_cuda, n, eps2):
     ke, pe = hhs.energy(d.pos, d.vel, d.m, eps2)
     assert abs(ke - oracle.kinetic(vel, m)) <= 1e-12 * abs(ke)
     assert abs(pe - oracle.potential(pos, m, eps2)) <= 1e-12 * abs(pe)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 65, 1024, 1025, 3000])
+@pytest.mark.parametrize("eps2", [0.0, 1e-8])
+def test_timesteps_symmetric_bit_exact(hhs, oracle, torch_cuda, n, eps2):
+    """Symmetric evaluation (each unordered pair once, partial minima then a
+    min-reduction): EXACT mode bit-exact with the oracle, including ragged
+    last blocks (1025) and a coincident pair with eps2 = 0."""
+    pos, vel, m = plummer(hhs, n, seed=640 + n, mass_model=1, eps2=eps2)
+    if n > 10:
+        pos[9] = pos[4]
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps_sym(d, eta=0.01, eps2=eps2, mode=hhs.TIMESTEP_EXACT)
+    torch_cuda.cuda.synchronize()
+    ref = oracle.timesteps(pos, vel, m, 0.01, eps2)
+    assert d.timestep.cpu().numpy().tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("n", [3000, 40000])
+def test_timesteps_symmetric_large(hhs, oracle, torch_cuda, n):
+    """At sizes HOLD runs it: EXACT equals the one-sided EXACT kernel (itself
+    bit-exact with the oracle) bit for bit; FP32 within 1e-6 of it."""
+    eps2 = 1e-8
+    pos, vel, m = plummer(hhs, n, seed=700 + n, mass_model=1, eps2=eps2)
+    a = hhs.DeviceParticles.from_host(pos, vel, m)
+    b = hhs.DeviceParticles.from_host(pos, vel, m)
+    c = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(a, eta=0.01, eps2=eps2)
+    hhs.timesteps_sym(b, eta=0.01, eps2=eps2, mode=hhs.TIMESTEP_EXACT)
+    hhs.timesteps_sym(c, eta=0.01, eps2=eps2, mode=hhs.TIMESTEP_FP32)
+    torch_cuda.cuda.synchronize()
+    ta, tb, tc = (x.timestep.cpu().numpy() for x in (a, b, c))
+    assert ta.tobytes() == tb.tobytes()
+    assert np.max(np.abs(tc / ta - 1.0)) <= 1e-6
