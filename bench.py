@@ -460,7 +460,7 @@ def main():
     tune = None
     if not args.no_tune:
         t0 = time.perf_counter()
-        s_t, w_t, ms_t = hhs.autotune_kick(N, 3)
+        s_t, w_t, ms_t = hhs.autotune_kick(N, 10)
         tune = {"sinks_per_lane": s_t, "waves_per_group": w_t, "ms": round(ms_t, 4),
                 "sweep_seconds": round(time.perf_counter() - t0, 2)}
         log(f"autotune: {tune}")
