@@ -361,7 +361,7 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
 
 
 def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5, batch_streams=0, big_workers=0,
-                small_max_n=0):
+                small_max_n=0, progress=False):
     """Served workload: Poisson arrivals of Plummer integrations with N
     log-uniform in [n_lo, n_hi], end time uniform in [0.1, 1], eta = 0.01,
     through the continuous-batching server; time-to-solution = completion -
@@ -387,7 +387,12 @@ def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5, batch_streams=0, big_work
                 if delay > 0:
                     time.sleep(delay)
                 tickets.append(srv.submit(p, v, m, te, 1.0 / 16))
-            infos = [srv.wait(t) for t in tickets]
+            infos = []
+            for k, t in enumerate(tickets):
+                infos.append(srv.wait(t))
+                if progress:
+                    log(f"served rate {rate}: request {k} (N={len(reqs[k][2])}) done after "
+                        f"{infos[-1]['t_done'] - infos[-1]['t_submit']:.2f} s")
             wall = time.perf_counter() - t0
             peak = srv.stats()["peak_bytes"]
         tts = np.array([i["t_done"] - i["t_submit"] for i in infos])
@@ -533,7 +538,7 @@ def main():
         log(f"ensemble: {out['ensemble']}")
     if not args.no_serve:
         if args.serve_full:
-            out["served"] = serve_bench(hhs, [0.05, 0.2], 32, 64, 65536)
+            out["served"] = serve_bench(hhs, [0.5, 2.0], 24, 64, 65536, progress=True)
         else:
             out["served"] = serve_bench(hhs, [4.0, 16.0], 48, 64, 4096)
             out["served"]["note"] = ("N range capped at 4096 for the default run; "

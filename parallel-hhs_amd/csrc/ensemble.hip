@@ -719,10 +719,14 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
 
     char *w = static_cast<char *>(ws);
     hipStream_t st = as_stream(stream);
+    // Ordered on `st`: the workspace may come from the stream-ordered
+    // allocator on that stream.  The sources are pageable, so each copy has
+    // read them before it returns.
     if (!big.empty())
-        HHS_HIP(hipMemcpy(w + L.big, big.data(), big.size() * 4, hipMemcpyHostToDevice));
+        HHS_HIP(hipMemcpyAsync(w + L.big, big.data(), big.size() * 4, hipMemcpyHostToDevice, st));
     if (!small.empty())
-        HHS_HIP(hipMemcpy(w + L.small, small.data(), small.size() * 4, hipMemcpyHostToDevice));
+        HHS_HIP(hipMemcpyAsync(w + L.small, small.data(), small.size() * 4, hipMemcpyHostToDevice,
+                               st));
     unsigned int *counters = reinterpret_cast<unsigned int *>(w + L.counters);
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, counters);
     HHS_LAUNCH_CHECK("zero_counters_kernel");
