@@ -24,6 +24,35 @@ int hip_status(hipError_t e, const char *what, const char *file, int line) {
     return HHS_ERR_HIP;
 }
 
+ThreadScratch::~ThreadScratch() {
+    if (done_) (void)hipEventDestroy(done_);
+    if (ptr_) (void)hipFree(ptr_);
+}
+
+int ThreadScratch::acquire(size_t bytes, hipStream_t st, void **out) {
+    if (!done_) HHS_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    if (bytes > bytes_) {
+        if (used_) HHS_HIP(hipEventSynchronize(done_));  // the last user is done
+        if (ptr_) HHS_HIP(hipFree(ptr_));
+        ptr_ = nullptr;
+        bytes_ = 0;
+        used_ = false;
+        HHS_HIP(hipMalloc(&ptr_, bytes));
+        bytes_ = bytes;
+    } else if (used_ && stream_ != st) {
+        HHS_HIP(hipStreamWaitEvent(st, done_, 0));
+    }
+    *out = ptr_;
+    return HHS_OK;
+}
+
+int ThreadScratch::release(hipStream_t st) {
+    HHS_HIP(hipEventRecord(done_, st));
+    stream_ = st;
+    used_ = true;
+    return HHS_OK;
+}
+
 }  // namespace hhs
 
 using namespace hhs;

@@ -17,6 +17,25 @@ void clear_error();
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Per-thread device scratch reused across launches, grown on demand.  A
+// launch on another stream than the previous user first waits (on the
+// device) for that user's completion event, so one host thread's launches
+// on several streams never use it concurrently.  Usage: acquire, queue the
+// kernels on `st`, release(st).
+class ThreadScratch {
+  public:
+    ~ThreadScratch();
+    int acquire(size_t bytes, hipStream_t st, void **out);
+    int release(hipStream_t st);
+
+  private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+    hipEvent_t done_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    bool used_ = false;
+};
+
 // Returns HHS_ERR_HIP (and records the message) when `e` is not hipSuccess.
 int hip_status(hipError_t e, const char *what, const char *file, int line);
 

@@ -154,15 +154,18 @@ template <int S, bool SOFT> struct KickAcc<double, S, 3, SOFT> {
     }
 };
 
-// One workgroup: 64*S sinks (S per lane), W waves splitting the source list
-// (segment 0 then segment 1), per-wave totals merged through LDS in wave
-// order (deterministic), epilogue acc <- a, vel += a * h.
+// One workgroup: 64*S sinks (S per lane) against source chunk blockIdx.y of
+// K (split-K: small sink counts get more waves over the same sources), W
+// waves splitting the chunk (segment 0 then segment 1), per-wave totals
+// merged through LDS in wave order (deterministic).  With K == 1 the
+// epilogue writes acc <- a, vel += a * h; otherwise the chunk's FP64 partial
+// sums go to partial[k][i][3] for kick_reduce_kernel.
 template <typename T, int S, int W, int P, bool SOFT>
 __global__ __launch_bounds__(64 * W) void kick_kernel(
     const T *__restrict__ si_pos, T *__restrict__ si_vel, T *__restrict__ si_acc, int n_sink,
     const T *__restrict__ s0_pos, const T *__restrict__ s0_m, int n0,
     const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1, T h, float eps2f,
-    double eps2) {
+    double eps2, int K, double *__restrict__ partial) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * 3 * S * 64];
 
     const int lane = threadIdx.x & 63;
@@ -179,11 +182,15 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
         zi[s] = si_pos[3 * i + 2];
     }
     KickAcc<T, S, P, SOFT> acc;
-    // This wave's contiguous share of the concatenated source list.
+    // This workgroup's chunk of the concatenated source list, then this
+    // wave's contiguous share of it.
     const int total = n0 + n1;
-    const int chunk = (total + W - 1) / W;
-    const int jb = min(wave * chunk, total);
-    const int je = min(jb + chunk, total);
+    const int per = (total + K - 1) / K;
+    const int kb = min(static_cast<int>(blockIdx.y) * per, total);
+    const int ke = min(kb + per, total);
+    const int chunk = (ke - kb + W - 1) / W;
+    const int jb = min(kb + wave * chunk, ke);
+    const int je = min(jb + chunk, ke);
     if (jb < n0) acc.range(s0_pos, s0_m, jb, min(je, n0), xi, yi, zi, eps2f, eps2);
     if (je > n0) acc.range(s1_pos, s1_m, max(jb, n0) - n0, je - n0, xi, yi, zi, eps2f, eps2);
 
@@ -212,6 +219,19 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
             }
         }
     }
+    if (partial) {
+        double *q = partial + 3 * static_cast<size_t>(blockIdx.y) * n_sink;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = tile0 + s * 64 + lane;
+            if (i < n_sink) {
+                q[3 * i + 0] = ax[s];
+                q[3 * i + 1] = ay[s];
+                q[3 * i + 2] = az[s];
+            }
+        }
+        return;
+    }
     // Epilogue (src/kick.cuh:98-104): acc <- a, vel += a * (fac * dt), as a
     // multiply then an add (no contraction, like the FP64 CPU update).
     {
@@ -235,41 +255,75 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     }
 }
 
+// Sum of the K chunk partials of each sink in chunk order (deterministic),
+// then the epilogue of kick_kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ si_vel,
+                                                          T *__restrict__ si_acc, int n_sink,
+                                                          const double *__restrict__ partial,
+                                                          int K, T h) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_sink) return;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const double *q = partial + 3 * (static_cast<size_t>(k) * n_sink + i);
+        ax += q[0];
+        ay += q[1];
+        az += q[2];
+    }
+    const T a0 = static_cast<T>(ax), a1 = static_cast<T>(ay), a2 = static_cast<T>(az);
+    if (si_acc) {
+        si_acc[3 * i + 0] = a0;
+        si_acc[3 * i + 1] = a1;
+        si_acc[3 * i + 2] = a2;
+    }
+    si_vel[3 * i + 0] = si_vel[3 * i + 0] + a0 * h;
+    si_vel[3 * i + 1] = si_vel[3 * i + 1] + a1 * h;
+    si_vel[3 * i + 2] = si_vel[3 * i + 2] + a2 * h;
+}
+
 template <typename T, int S, int W, int P>
-int launch_swp(const KickArgs<T> &a, hipStream_t st) {
+int launch_swp(const KickArgs<T> &a, int K, double *partial, hipStream_t st) {
     const int tile = 64 * S;
-    dim3 grid((a.n_sink + tile - 1) / tile), block(64 * W);
+    dim3 grid((a.n_sink + tile - 1) / tile, K), block(64 * W);
     const double eps2 = a.eps2;
     const float eps2f = static_cast<float>(a.eps2);
+    double *part = K > 1 ? partial : nullptr;
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, true>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2);
+                           a.n1, a.h, eps2f, eps2, K, part);
     else
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, false>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2);
+                           a.n1, a.h, eps2f, eps2, K, part);
     HHS_LAUNCH_CHECK("kick_kernel");
+    if (K > 1) {
+        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((a.n_sink + 255) / 256), dim3(256), 0, st,
+                           a.si_vel, a.si_acc, a.n_sink, part, K, a.h);
+        HHS_LAUNCH_CHECK("kick_reduce_kernel");
+    }
     return HHS_OK;
 }
 
 template <typename T, int S, int W>
-int launch_sw(const KickArgs<T> &a, int P, hipStream_t st) {
+int launch_sw(const KickArgs<T> &a, int P, int K, double *partial, hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
-        if (P == 1) return launch_swp<T, S, W, 1>(a, st);
-        if (P == 2) return launch_swp<T, S, W, 2>(a, st);
-        if (P == 3) return launch_swp<T, S, W, 3>(a, st);
+        if (P == 1) return launch_swp<T, S, W, 1>(a, K, partial, st);
+        if (P == 2) return launch_swp<T, S, W, 2>(a, K, partial, st);
+        if (P == 3) return launch_swp<T, S, W, 3>(a, K, partial, st);
     }
-    return launch_swp<T, S, W, 0>(a, st);
+    return launch_swp<T, S, W, 0>(a, K, partial, st);
 }
 
 template <typename T, int S>
-int launch_s(const KickArgs<T> &a, int W, int P, hipStream_t st) {
+int launch_s(const KickArgs<T> &a, int W, int P, int K, double *partial, hipStream_t st) {
     switch (W) {
-    case 1: return launch_sw<T, S, 1>(a, P, st);
-    case 2: return launch_sw<T, S, 2>(a, P, st);
-    case 4: return launch_sw<T, S, 4>(a, P, st);
-    case 8: return launch_sw<T, S, 8>(a, P, st);
+    case 1: return launch_sw<T, S, 1>(a, P, K, partial, st);
+    case 2: return launch_sw<T, S, 2>(a, P, K, partial, st);
+    case 4: return launch_sw<T, S, 4>(a, P, K, partial, st);
+    case 8: return launch_sw<T, S, 8>(a, P, K, partial, st);
     }
     set_error("kick: unsupported waves per group %d", W);
     return HHS_ERR_INVALID;
@@ -311,6 +365,25 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
     return c;
 }
 
+// Split-K for small sink counts: more workgroups over chunks of the same
+// source list while the grid stays under 8192 waves (8 per SIMD on 256
+// CUs), each wave keeps >= 32 sources and the FP64 partials (24 B per sink
+// and chunk) stay within kMaxPartialBytes.  At N = 4096 a plain launch is
+// 512 waves, each running 512 sources back to back with nothing to hide
+// its latency.  MI355X, square kicks: N = 4096 48 -> 21 us, 8192 96 ->
+// 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.
+constexpr size_t kMaxPartialBytes = size_t{4} << 20;
+
+int pick_split(int n_sink, int n_src, int S, int W) {
+    const long long tiles = (n_sink + 64LL * S - 1) / (64LL * S);
+    int K = 1;
+    if (n_src < 2048) return K;  // the reduce launch would cost more than it saves
+    while (K < 64 && tiles * W * K * 2 <= 8192 && n_src / (W * K * 2) >= 32 &&
+           24ULL * n_sink * K * 2 <= kMaxPartialBytes)
+        K *= 2;
+    return K;
+}
+
 template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     HHS_REQUIRE(a.n_sink >= 0 && a.n0 >= 0 && a.n1 >= 0, "kick: negative size");
     if (a.n_sink == 0) return HHS_OK;
@@ -326,13 +399,26 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
         c.sinks_per_lane = 1;
         c.waves = 1;
     }
-    switch (c.sinks_per_lane) {
-    case 1: return launch_s<T, 1>(a, c.waves, P, st);
-    case 2: return launch_s<T, 2>(a, c.waves, P, st);
-    case 4: return launch_s<T, 4>(a, c.waves, P, st);
+    // The exact mode keeps one sequential sum per sink.
+    const int K = (P == HHS_KICK_F64_EXACT) ? 1 : pick_split(a.n_sink, a.n0 + a.n1,
+                                                             c.sinks_per_lane, c.waves);
+    static thread_local ThreadScratch scratch;
+    double *partial = nullptr;
+    if (K > 1) {
+        void *p = nullptr;
+        int rc = scratch.acquire(24 * static_cast<size_t>(a.n_sink) * K, st, &p);
+        if (rc) return rc;
+        partial = static_cast<double *>(p);
     }
-    set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
-    return HHS_ERR_INVALID;
+    int rc = HHS_ERR_INVALID;
+    switch (c.sinks_per_lane) {
+    case 1: rc = launch_s<T, 1>(a, c.waves, P, K, partial, st); break;
+    case 2: rc = launch_s<T, 2>(a, c.waves, P, K, partial, st); break;
+    case 4: rc = launch_s<T, 4>(a, c.waves, P, K, partial, st); break;
+    default: set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
+    }
+    if (rc == HHS_OK && K > 1) rc = scratch.release(st);
+    return rc;
 }
 
 template int launch_kick<double>(const KickArgs<double> &, hipStream_t);

@@ -25,7 +25,9 @@ def _check(a_gpu, a_ref, scale):
     return float(err.max())
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257, 1000, 2048])
+# Sizes below ~32768 split the sources over several workgroups (split-K)
+# with FP64 partials summed in chunk order; 5000 exercises a ragged split.
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 257, 1000, 2048, 5000])
 @pytest.mark.parametrize("eps2", [0.0, 1e-8])
 def test_kick_self_matches_oracle(hhs, oracle, torch_cuda, n, eps2):
     pos, vel, m = plummer(hhs, n, seed=100 + n, eps2=eps2)
