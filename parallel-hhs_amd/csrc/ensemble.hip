@@ -31,6 +31,7 @@ namespace hhs {
 namespace {
 
 constexpr int kWG = 256;
+constexpr int kWaves = kWG / 64;
 constexpr int kMaxFrames = kSubtreeLevels;  // max_level <= 47 plus root and one child
 constexpr int kFold = 64;
 
@@ -45,7 +46,7 @@ struct TeamShared {
     int level_calls[kMaxFrames];  // calls per recursion level (relative)
     int sp;
     int status;
-    int wave_tot[4];
+    int wave_tot[kWaves];
     double red[kWG];
 };
 
@@ -105,7 +106,9 @@ __device__ int team_sum(int x, TeamShared &sh, const Team<TEAM> &tm) {
     } else {
         if ((tm.t & 63) == 0) sh.wave_tot[tm.t >> 6] = x;
         tm.sync();
-        const int s = sh.wave_tot[0] + sh.wave_tot[1] + sh.wave_tot[2] + sh.wave_tot[3];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) s += sh.wave_tot[k];
         tm.sync();
         return s;
     }
@@ -129,7 +132,7 @@ __device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared &sh,
         tm.sync();
         int before = 0, all = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kWaves; ++k) {
             const int c = sh.wave_tot[k];
             before += (k < w) ? c : 0;
             all += c;
@@ -187,57 +190,113 @@ __device__ __forceinline__ void load_ts_tile(const EnsArgs &a, int s0, int je, d
     tm.sync();
 }
 
+// One pass over the sources [jb, je) (index order, through LDS tiles) for S
+// sinks per lane: sinks i0 + t + s * TEAM.  Each sink's arithmetic and
+// source order are those of one sink per lane, so the result does not
+// depend on S; several sinks per lane give each wave independent chains to
+// interleave, which a team running alone on its CU needs.
+template <int TEAM, int S, int MODE>
+__device__ void team_timesteps_pass(const EnsArgs &a, int jb, int je, int i0, double *tile,
+                                    const Team<TEAM> &tm) {
+    int ii[S];
+    double xi[S], yi[S], zi[S], vxi[S], vyi[S], vzi[S], mi[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        ii[s] = i0 + tm.t + s * TEAM;
+        const int ic = ii[s] < je ? ii[s] : je - 1;
+        xi[s] = a.wpos[3 * ic];
+        yi[s] = a.wpos[3 * ic + 1];
+        zi[s] = a.wpos[3 * ic + 2];
+        vxi[s] = a.wvel[3 * ic];
+        vyi[s] = a.wvel[3 * ic + 1];
+        vzi[s] = a.wvel[3 * ic + 2];
+        mi[s] = a.wm[ic];
+    }
+    double minval[S], bden[S];
+    float bnum32[S], bden32[S], mif[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        minval[s] = TsConst<double>::huge;
+        bden[s] = 1.0;
+        bnum32[s] = static_cast<float>(TsConst<double>::huge);
+        bden32[s] = 1.0f;
+        mif[s] = static_cast<float>(mi[s]);
+    }
+    const float etaf = static_cast<float>(a.eta), cff = static_cast<float>(a.c_ff);
+    for (int s0 = jb; s0 < je; s0 += TEAM) {
+        load_ts_tile<TEAM>(a, s0, je, tile, tm);
+        const int cnt = uni(min(TEAM, je - s0));
+#pragma unroll 2
+        for (int k = 0; k < cnt; ++k) {
+            const double *q = tile + kTsStride * k;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const bool other = s0 + k != ii[s];
+                if constexpr (MODE == HHS_TIMESTEP_FP32)
+                    pair_timestep_f32(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mif[s], q[0],
+                                      q[1], q[2], q[4], q[5], q[6], static_cast<float>(q[3]), etaf,
+                                      cff, a.eps2f, other, bnum32[s], bden32[s]);
+                else if constexpr (MODE == HHS_TIMESTEP_FAST)
+                    pair_timestep_fast(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s], q[0],
+                                       q[1], q[2], q[4], q[5], q[6], q[3], a.eta, a.c_ff, a.eps2,
+                                       other, minval[s], bden[s]);
+                else
+                    pair_timestep<double>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s], q[0],
+                                          q[1], q[2], q[4], q[5], q[6], q[3], a.eta, a.c_ff,
+                                          a.eps2, other, minval[s]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double v = minval[s];
+        if constexpr (MODE == HHS_TIMESTEP_FP32)
+            v = static_cast<double>(bnum32[s]) / static_cast<double>(bden32[s]);
+        else if constexpr (MODE == HHS_TIMESTEP_FAST)
+            v = minval[s] / bden[s];
+        if (ii[s] < je) a.wts[ii[s]] = v;
+    }
+}
+
+// Sinks per lane for a pass with `rem` sinks left: enough to cover them,
+// at most kMaxSpl (FP32 timesteps and kicks; the FP64 timestep modes keep
+// one, their pair state would cost the kernel a wave of occupancy).
+constexpr int kMaxSpl = 4;
+template <int TEAM> __device__ __forceinline__ int sinks_per_lane(int rem, int cap) {
+    return min(cap, (rem + TEAM - 1) / TEAM);
+}
+
+template <int TEAM, int MODE>
+__device__ void team_timesteps_mode(const EnsArgs &a, int jb, int je, double *tile,
+                                    const Team<TEAM> &tm) {
+    constexpr int cap = MODE == HHS_TIMESTEP_FP32 ? kMaxSpl : 1;
+    for (int i0 = jb; i0 < je;) {
+        const int S = sinks_per_lane<TEAM>(je - i0, cap);
+        if (S >= 4) {
+            team_timesteps_pass<TEAM, (cap >= 4 ? 4 : 1), MODE>(a, jb, je, i0, tile, tm);
+        } else if (S == 3) {
+            team_timesteps_pass<TEAM, (cap >= 3 ? 3 : 1), MODE>(a, jb, je, i0, tile, tm);
+        } else if (S == 2) {
+            team_timesteps_pass<TEAM, (cap >= 2 ? 2 : 1), MODE>(a, jb, je, i0, tile, tm);
+        } else {
+            team_timesteps_pass<TEAM, 1, MODE>(a, jb, je, i0, tile, tm);
+        }
+        i0 += S * TEAM;
+    }
+}
+
 // timesteps of particles [o+b, o+e) against each other (src/timestep.cuh),
 // sources in index order through LDS tiles.
 template <int TEAM>
 __device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, double *tile,
                                const Team<TEAM> &tm) {
     const int jb = uni(o + b), je = uni(o + e);
-    for (int i0 = jb; i0 < je; i0 += TEAM) {
-        const int i = i0 + tm.t;
-        const int ic = i < je ? i : je - 1;
-        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
-        const double vxi = a.wvel[3 * ic], vyi = a.wvel[3 * ic + 1], vzi = a.wvel[3 * ic + 2];
-        const double mi = a.wm[ic];
-        double minval = TsConst<double>::huge, bden = 1.0;
-        float bnum32 = static_cast<float>(TsConst<double>::huge), bden32 = 1.0f;
-        const float mif = static_cast<float>(mi), etaf = static_cast<float>(a.eta),
-                    cff = static_cast<float>(a.c_ff);
-        for (int s0 = jb; s0 < je; s0 += TEAM) {
-            load_ts_tile<TEAM>(a, s0, je, tile, tm);
-            const int cnt = uni(min(TEAM, je - s0));
-            if (a.ts_mode == HHS_TIMESTEP_FP32) {
-#pragma unroll 2
-                for (int k = 0; k < cnt; ++k) {
-                    const double *q = tile + kTsStride * k;
-                    pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mif, q[0], q[1], q[2], q[4], q[5],
-                                      q[6], static_cast<float>(q[3]), etaf, cff, a.eps2f,
-                                      s0 + k != i, bnum32, bden32);
-                }
-            } else if (a.ts_mode == HHS_TIMESTEP_FAST) {
-#pragma unroll 2
-                for (int k = 0; k < cnt; ++k) {
-                    const double *q = tile + kTsStride * k;
-                    pair_timestep_fast(xi, yi, zi, vxi, vyi, vzi, mi, q[0], q[1], q[2], q[4], q[5],
-                                       q[6], q[3], a.eta, a.c_ff, a.eps2, s0 + k != i, minval,
-                                       bden);
-                }
-            } else {
-#pragma unroll 2
-                for (int k = 0; k < cnt; ++k) {
-                    const double *q = tile + kTsStride * k;
-                    pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, q[0], q[1], q[2], q[4],
-                                          q[5], q[6], q[3], a.eta, a.c_ff, a.eps2, s0 + k != i,
-                                          minval);
-                }
-            }
-        }
-        if (a.ts_mode == HHS_TIMESTEP_FP32)
-            minval = static_cast<double>(bnum32) / static_cast<double>(bden32);
-        else if (a.ts_mode == HHS_TIMESTEP_FAST)
-            minval = minval / bden;
-        if (i < je) a.wts[i] = minval;
-    }
+    if (a.ts_mode == HHS_TIMESTEP_FP32)
+        team_timesteps_mode<TEAM, HHS_TIMESTEP_FP32>(a, jb, je, tile, tm);
+    else if (a.ts_mode == HHS_TIMESTEP_FAST)
+        team_timesteps_mode<TEAM, HHS_TIMESTEP_FAST>(a, jb, je, tile, tm);
+    else
+        team_timesteps_mode<TEAM, HHS_TIMESTEP_EXACT>(a, jb, je, tile, tm);
 }
 
 // Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
@@ -277,6 +336,71 @@ __device__ void team_drift(const EnsArgs &a, int o, int b, int e, double h,
         a.wpos[k] = a.wpos[k] + a.wvel[k] * h;  // src/drift.cuh:25
 }
 
+// One pass of team_kick for S sinks per lane (sinks i0 + t + s * TEAM); the
+// per-sink sum is the same for every S.
+template <int TEAM, int S, bool SOFT>
+__device__ void team_kick_pass(const EnsArgs &a, int i0, int se, int jb, int je, double h,
+                               double *tile, const Team<TEAM> &tm) {
+    int ii[S];
+    double xi[S], yi[S], zi[S];
+    float hx[S], lx[S], hy[S], ly[S], hz[S], lz[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        ii[s] = i0 + tm.t + s * TEAM;
+        const int ic = ii[s] < se ? ii[s] : se - 1;
+        xi[s] = a.wpos[3 * ic];
+        yi[s] = a.wpos[3 * ic + 1];
+        zi[s] = a.wpos[3 * ic + 2];
+        hx[s] = lx[s] = hy[s] = ly[s] = hz[s] = lz[s] = 0.f;
+    }
+    for (int s0 = jb; s0 < je; s0 += TEAM) {
+        tm.sync();  // the previous tile has been read by every lane
+        const int j = s0 + tm.t;
+        if (j < je) {
+            double *q = tile + kKickStride * tm.t;
+            q[0] = a.wpos[3 * j];
+            q[1] = a.wpos[3 * j + 1];
+            q[2] = a.wpos[3 * j + 2];
+            q[3] = a.wm[j];
+        }
+        tm.sync();
+        const int cnt = uni(min(TEAM, je - s0));
+        for (int k0 = 0; k0 < cnt; k0 += kFold) {
+            const int k1 = min(k0 + kFold, cnt);
+            float px[S], py[S], pz[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) px[s] = py[s] = pz[s] = 0.f;
+#pragma unroll 4
+            for (int k = k0; k < k1; ++k) {
+                const double *q = tile + kKickStride * k;
+                const float mj = static_cast<float>(q[3]);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    pair_accel<SOFT>(q[0], q[1], q[2], mj, xi[s], yi[s], zi[s], a.eps2f, px[s],
+                                     py[s], pz[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                two_sum_add(hx[s], lx[s], px[s]);
+                two_sum_add(hy[s], ly[s], py[s]);
+                two_sum_add(hz[s], lz[s], pz[s]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = ii[s];
+        if (i < se) {
+            const double ax = static_cast<double>(hx[s]) + static_cast<double>(lx[s]);
+            const double ay = static_cast<double>(hy[s]) + static_cast<double>(ly[s]);
+            const double az = static_cast<double>(hz[s]) + static_cast<double>(lz[s]);
+            a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
+            a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
+            a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
+        }
+    }
+}
+
 // vel[i] += h * sum_j m_j (x_j - x_i) / (r^2 + eps2)^{3/2}, sinks [sb, se),
 // sources [jb, je) (absolute indices) in index order through LDS tiles,
 // summed in FP32 over blocks of kFold sources from jb (TEAM is a multiple of
@@ -288,45 +412,17 @@ __device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, doub
     static_assert(TEAM % kFold == 0, "tiles must hold whole fold blocks");
     jb = uni(jb);
     je = uni(je);
-    for (int i0 = uni(sb); i0 < se; i0 += TEAM) {
-        const int i = i0 + tm.t;
-        const int ic = i < se ? i : se - 1;
-        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
-        float hx = 0.f, lx = 0.f, hy = 0.f, ly = 0.f, hz = 0.f, lz = 0.f;
-        for (int s0 = jb; s0 < je; s0 += TEAM) {
-            tm.sync();  // the previous tile has been read by every lane
-            const int j = s0 + tm.t;
-            if (j < je) {
-                double *q = tile + kKickStride * tm.t;
-                q[0] = a.wpos[3 * j];
-                q[1] = a.wpos[3 * j + 1];
-                q[2] = a.wpos[3 * j + 2];
-                q[3] = a.wm[j];
-            }
-            tm.sync();
-            const int cnt = uni(min(TEAM, je - s0));
-            for (int k0 = 0; k0 < cnt; k0 += kFold) {
-                const int k1 = min(k0 + kFold, cnt);
-                float px = 0.f, py = 0.f, pz = 0.f;
-#pragma unroll 4
-                for (int k = k0; k < k1; ++k) {
-                    const double *q = tile + kKickStride * k;
-                    pair_accel<SOFT>(q[0], q[1], q[2], static_cast<float>(q[3]), xi, yi, zi,
-                                     a.eps2f, px, py, pz);
-                }
-                two_sum_add(hx, lx, px);
-                two_sum_add(hy, ly, py);
-                two_sum_add(hz, lz, pz);
-            }
-        }
-        if (i < se) {
-            const double ax = static_cast<double>(hx) + static_cast<double>(lx);
-            const double ay = static_cast<double>(hy) + static_cast<double>(ly);
-            const double az = static_cast<double>(hz) + static_cast<double>(lz);
-            a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
-            a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
-            a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
-        }
+    for (int i0 = uni(sb); i0 < se;) {
+        const int S = sinks_per_lane<TEAM>(se - i0, kMaxSpl);
+        if (S >= 4)
+            team_kick_pass<TEAM, 4, SOFT>(a, i0, se, jb, je, h, tile, tm);
+        else if (S == 3)
+            team_kick_pass<TEAM, 3, SOFT>(a, i0, se, jb, je, h, tile, tm);
+        else if (S == 2)
+            team_kick_pass<TEAM, 2, SOFT>(a, i0, se, jb, je, h, tile, tm);
+        else
+            team_kick_pass<TEAM, 1, SOFT>(a, i0, se, jb, je, h, tile, tm);
+        i0 += S * TEAM;
     }
 }
 
@@ -511,7 +607,7 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
 
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
-    __shared__ TeamShared sh[4];
+    __shared__ TeamShared sh[kWaves];
     // Source tiles: the whole array for a workgroup team (kWG sources), a
     // quarter per wave team (64 sources each).
     __shared__ double tiles[kWG * kTsStride];

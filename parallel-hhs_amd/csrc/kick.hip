@@ -51,9 +51,21 @@ template <typename T, int S, bool SOFT> struct KickAcc<T, S, 0, SOFT> {
 #pragma unroll
         for (int s = 0; s < S; ++s) hx[s] = lx[s] = hy[s] = ly[s] = hz[s] = lz[s] = 0.0f;
     }
-    __device__ void range(const T *__restrict__ sp, const T *__restrict__ sm, int jb, int je,
-                          const T (&xi)[S], const T (&yi)[S], const T (&zi)[S], float eps2f,
-                          double) {
+    // mf: null, or the source masses converted to FP32 up front (the loop
+    // then spends no VALU slot per source on the conversion); the values are
+    // the same either way.
+    __device__ void range(const T *__restrict__ sp, const T *__restrict__ sm, const float *mf,
+                          int jb, int je, const T (&xi)[S], const T (&yi)[S], const T (&zi)[S],
+                          float eps2f, double) {
+        if (mf)
+            fold<true>(sp, sm, mf, jb, je, xi, yi, zi, eps2f);
+        else
+            fold<false>(sp, sm, mf, jb, je, xi, yi, zi, eps2f);
+    }
+    template <bool PRE>
+    __device__ void fold(const T *__restrict__ sp, const T *__restrict__ sm,
+                         const float *__restrict__ mf, int jb, int je, const T (&xi)[S],
+                         const T (&yi)[S], const T (&zi)[S], float eps2f) {
         for (int j0 = jb; j0 < je; j0 += kFold) {
             const int j1 = min(j0 + kFold, je);
             float px[S], py[S], pz[S];
@@ -62,7 +74,7 @@ template <typename T, int S, bool SOFT> struct KickAcc<T, S, 0, SOFT> {
 #pragma unroll 4
             for (int j = j0; j < j1; ++j) {
                 const T xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
-                const float mj = static_cast<float>(sm[j]);
+                const float mj = PRE ? mf[j] : static_cast<float>(sm[j]);
 #pragma unroll
                 for (int s = 0; s < S; ++s)
                     pair_accel<SOFT>(xj, yj, zj, mj, xi[s], yi[s], zi[s], eps2f, px[s], py[s],
@@ -89,9 +101,9 @@ template <int S, int P, bool SOFT> struct KickAcc64 {
 #pragma unroll
         for (int s = 0; s < S; ++s) ax[s] = ay[s] = az[s] = 0.0;
     }
-    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm, int jb,
-                          int je, const double (&xi)[S], const double (&yi)[S],
-                          const double (&zi)[S], float, double eps2) {
+    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm,
+                          const float *, int jb, int je, const double (&xi)[S],
+                          const double (&yi)[S], const double (&zi)[S], float, double eps2) {
 #pragma unroll 4
         for (int j = jb; j < je; ++j) {
             const double xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
@@ -135,9 +147,9 @@ template <int S, bool SOFT> struct KickAcc<double, S, 3, SOFT> {
 #pragma unroll
         for (int s = 0; s < S; ++s) ax[s] = ay[s] = az[s] = 0.0;
     }
-    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm, int jb,
-                          int je, const double (&xi)[S], const double (&yi)[S],
-                          const double (&zi)[S], float, double eps2) {
+    __device__ void range(const double *__restrict__ sp, const double *__restrict__ sm,
+                          const float *, int jb, int je, const double (&xi)[S],
+                          const double (&yi)[S], const double (&zi)[S], float, double eps2) {
 #pragma unroll 2
         for (int j = jb; j < je; ++j) {
             const double xj = sp[3 * j + 0], yj = sp[3 * j + 1], zj = sp[3 * j + 2];
@@ -165,7 +177,7 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     const T *__restrict__ si_pos, T *__restrict__ si_vel, T *__restrict__ si_acc, int n_sink,
     const T *__restrict__ s0_pos, const T *__restrict__ s0_m, int n0,
     const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1, T h, float eps2f,
-    double eps2, int K, double *__restrict__ partial) {
+    double eps2, int K, double *__restrict__ partial, const float *__restrict__ mf) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * 3 * S * 64];
 
     const int lane = threadIdx.x & 63;
@@ -191,8 +203,11 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     const int chunk = (ke - kb + W - 1) / W;
     const int jb = min(kb + wave * chunk, ke);
     const int je = min(jb + chunk, ke);
-    if (jb < n0) acc.range(s0_pos, s0_m, jb, min(je, n0), xi, yi, zi, eps2f, eps2);
-    if (je > n0) acc.range(s1_pos, s1_m, max(jb, n0) - n0, je - n0, xi, yi, zi, eps2f, eps2);
+    // FP32 masses converted up front (segment 1 follows segment 0), or null.
+    const float *mf0 = mf, *mf1 = mf ? mf + n0 : nullptr;
+    if (jb < n0) acc.range(s0_pos, s0_m, mf0, jb, min(je, n0), xi, yi, zi, eps2f, eps2);
+    if (je > n0)
+        acc.range(s1_pos, s1_m, mf1, max(jb, n0) - n0, je - n0, xi, yi, zi, eps2f, eps2);
 
     double ax[S], ay[S], az[S];
 #pragma unroll
@@ -255,6 +270,23 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     }
 }
 
+// Source masses of both segments, rounded once to FP32 (segment 1 after
+// segment 0).
+constexpr double kConvMinWork = 536870912.0; // 2^29 interactions
+
+// Source masses of both segments rounded once to FP32 (segment 1 after
+// segment 0).
+template <typename T>
+__global__ __launch_bounds__(256) void masses_to_f32_kernel(const T *__restrict__ m0, int n0,
+                                                            const T *__restrict__ m1, int n1,
+                                                            float *__restrict__ out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n0)
+        out[j] = static_cast<float>(m0[j]);
+    else if (j < n0 + n1)
+        out[j] = static_cast<float>(m1[j - n0]);
+}
+
 // Sum of the K chunk partials of each sink in chunk order (deterministic),
 // then the epilogue of kick_kernel.
 template <typename T>
@@ -284,7 +316,7 @@ __global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ si_vel
 }
 
 template <typename T, int S, int W, int P>
-int launch_swp(const KickArgs<T> &a, int K, double *partial, hipStream_t st) {
+int launch_swp(const KickArgs<T> &a, int K, double *partial, const float *mf, hipStream_t st) {
     const int tile = 64 * S;
     dim3 grid((a.n_sink + tile - 1) / tile, K), block(64 * W);
     const double eps2 = a.eps2;
@@ -293,11 +325,11 @@ int launch_swp(const KickArgs<T> &a, int K, double *partial, hipStream_t st) {
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, true>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2, K, part);
+                           a.n1, a.h, eps2f, eps2, K, part, mf);
     else
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, false>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2, K, part);
+                           a.n1, a.h, eps2f, eps2, K, part, mf);
     HHS_LAUNCH_CHECK("kick_kernel");
     if (K > 1) {
         hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((a.n_sink + 255) / 256), dim3(256), 0, st,
@@ -308,22 +340,24 @@ int launch_swp(const KickArgs<T> &a, int K, double *partial, hipStream_t st) {
 }
 
 template <typename T, int S, int W>
-int launch_sw(const KickArgs<T> &a, int P, int K, double *partial, hipStream_t st) {
+int launch_sw(const KickArgs<T> &a, int P, int K, double *partial, const float *mf,
+              hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
-        if (P == 1) return launch_swp<T, S, W, 1>(a, K, partial, st);
-        if (P == 2) return launch_swp<T, S, W, 2>(a, K, partial, st);
-        if (P == 3) return launch_swp<T, S, W, 3>(a, K, partial, st);
+        if (P == 1) return launch_swp<T, S, W, 1>(a, K, partial, mf, st);
+        if (P == 2) return launch_swp<T, S, W, 2>(a, K, partial, mf, st);
+        if (P == 3) return launch_swp<T, S, W, 3>(a, K, partial, mf, st);
     }
-    return launch_swp<T, S, W, 0>(a, K, partial, st);
+    return launch_swp<T, S, W, 0>(a, K, partial, mf, st);
 }
 
 template <typename T, int S>
-int launch_s(const KickArgs<T> &a, int W, int P, int K, double *partial, hipStream_t st) {
+int launch_s(const KickArgs<T> &a, int W, int P, int K, double *partial, const float *mf,
+             hipStream_t st) {
     switch (W) {
-    case 1: return launch_sw<T, S, 1>(a, P, K, partial, st);
-    case 2: return launch_sw<T, S, 2>(a, P, K, partial, st);
-    case 4: return launch_sw<T, S, 4>(a, P, K, partial, st);
-    case 8: return launch_sw<T, S, 8>(a, P, K, partial, st);
+    case 1: return launch_sw<T, S, 1>(a, P, K, partial, mf, st);
+    case 2: return launch_sw<T, S, 2>(a, P, K, partial, mf, st);
+    case 4: return launch_sw<T, S, 4>(a, P, K, partial, mf, st);
+    case 8: return launch_sw<T, S, 8>(a, P, K, partial, mf, st);
     }
     set_error("kick: unsupported waves per group %d", W);
     return HHS_ERR_INVALID;
@@ -402,7 +436,7 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     // The exact mode keeps one sequential sum per sink.
     const int K = (P == HHS_KICK_F64_EXACT) ? 1 : pick_split(a.n_sink, a.n0 + a.n1,
                                                              c.sinks_per_lane, c.waves);
-    static thread_local ThreadScratch scratch;
+    static thread_local ThreadScratch scratch, mass_scratch;
     double *partial = nullptr;
     if (K > 1) {
         void *p = nullptr;
@@ -410,14 +444,32 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
         if (rc) return rc;
         partial = static_cast<double *>(p);
     }
+    // MIXED32 on FP64 storage, large launches: the source masses are
+    // converted to FP32 once up front (the same values the loop would convert
+    // per source and wave, one VALU op in 17); below ~2^29 interactions the
+    // extra launch costs more than it saves.
+    float *mf = nullptr;
+    const int n_src = a.n0 + a.n1;
+    const bool conv = std::is_same<T, double>::value && P == HHS_KICK_MIXED32 &&
+                      static_cast<double>(a.n_sink) * n_src >= kConvMinWork;
+    if (conv) {
+        void *p = nullptr;
+        int rc = mass_scratch.acquire(4 * static_cast<size_t>(n_src), st, &p);
+        if (rc) return rc;
+        mf = static_cast<float *>(p);
+        hipLaunchKernelGGL((masses_to_f32_kernel<T>), dim3((n_src + 255) / 256), dim3(256), 0, st,
+                           a.s0_m, a.n0, a.s1_m, a.n1, mf);
+        HHS_LAUNCH_CHECK("masses_to_f32_kernel");
+    }
     int rc = HHS_ERR_INVALID;
     switch (c.sinks_per_lane) {
-    case 1: rc = launch_s<T, 1>(a, c.waves, P, K, partial, st); break;
-    case 2: rc = launch_s<T, 2>(a, c.waves, P, K, partial, st); break;
-    case 4: rc = launch_s<T, 4>(a, c.waves, P, K, partial, st); break;
+    case 1: rc = launch_s<T, 1>(a, c.waves, P, K, partial, mf, st); break;
+    case 2: rc = launch_s<T, 2>(a, c.waves, P, K, partial, mf, st); break;
+    case 4: rc = launch_s<T, 4>(a, c.waves, P, K, partial, mf, st); break;
     default: set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
     }
     if (rc == HHS_OK && K > 1) rc = scratch.release(st);
+    if (rc == HHS_OK && conv) rc = mass_scratch.release(st);
     return rc;
 }
 

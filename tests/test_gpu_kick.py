@@ -182,6 +182,33 @@ def test_kick_n16384_sampled(hhs, oracle, torch_cuda):
     assert err < TOL, f"{err:.3e}"
 
 
+# From 2^29 interactions per launch the source masses are converted to FP32
+# in a separate pass before the force loop: large self and cross kicks, with
+# close pairs (1e-7 apart) and far outliers (|x| ~ 1e3), on sampled sinks.
+@pytest.mark.parametrize("n_sink,n_src,eps2", [(32768, 32768, 0.0), (32768, 32768, 1e-8),
+                                               (4096, 131072, 1e-8)])
+def test_kick_large_launch_sampled(hhs, oracle, torch_cuda, n_sink, n_src, eps2):
+    rng = np.random.default_rng(n_sink + n_src)
+    pos, vel, m = plummer(hhs, n_src, seed=7, eps2=eps2)
+    pos[1] = pos[0] + 1e-7 * rng.standard_normal(3)
+    pos[2] = pos[0] + np.array([1e-7, 0.0, 0.0])
+    pos[3:6] *= 1e3
+    src = _dev(hhs, pos, vel, m)
+    if n_sink == n_src:
+        hhs.kick_self(src, eps2=eps2)
+        si, spos = src, pos
+    else:
+        spos = pos[:n_sink].copy()
+        si = _dev(hhs, spos, vel[:n_sink].copy(), m[:n_sink].copy())
+        hhs.kick_cross(src, si, eps2=eps2)
+    torch_cuda.cuda.synchronize()
+    idx = np.concatenate([np.arange(8), rng.choice(np.arange(8, n_sink), 504, replace=False)])
+    ref = oracle.accel_subset(spos, idx, pos, m, eps2)
+    # sink i sits at source i's position, so the self-kick scale applies
+    err = _check(si.acc.cpu().numpy()[idx], ref, force_scale(pos, m, eps2, idx))
+    assert err < TOL, f"{err:.3e}"
+
+
 def test_kick_deterministic(hhs, torch_cuda):
     pos, vel, m = plummer(hhs, 3000, seed=9, eps2=1e-8)
     outs = []
