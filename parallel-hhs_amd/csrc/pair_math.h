@@ -163,7 +163,9 @@ __device__ __forceinline__ T pair_tau(T xi, T yi, T zi, T vxi, T vyi, T vzi, T m
 // (The one-sided kernels keep the fraction fold: with one sink per lane
 // the cross-multiplied compare is cheaper than two quarter-rate
 // reciprocals, 21.7 vs 24.8 ms at n = 131072; here it is the other way,
-// 15.1 vs 14.3 ms.)
+// 15.1 vs 14.3 ms.)  SOFT: eps2 > 0 in FP32, so r^2 > 0 and the r = 0 case
+// needs no test.
+template <bool SOFT>
 __device__ __forceinline__ float pair_tau_f32(double xi, double yi, double zi, double vxi,
                                               double vyi, double vzi, float mi, double xj,
                                               double yj, double zj, double vxj, double vyj,
@@ -178,7 +180,7 @@ __device__ __forceinline__ float pair_tau_f32(double xi, double yi, double zi, d
     const float rv = __builtin_fmaf(dvx, drx, __builtin_fmaf(dvy, dry, dvz * drz));
     const float mu = mj + mi;
     const float rinv = __builtin_amdgcn_rsqf(dr2);
-    const float r = dr2 > 0.0f ? dr2 * rinv : 0.0f;
+    const float r = (SOFT || dr2 > 0.0f) ? dr2 * rinv : 0.0f;
     const float rinv2 = rinv * rinv;
     const float u = rv * rinv2;
     // fminf(NaN, 1) = 1, as the (d < 1) ? d : 1 cap

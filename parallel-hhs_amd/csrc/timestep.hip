@@ -233,13 +233,13 @@ template <typename V> __device__ __forceinline__ V wave_min(V x) {
     return x;
 }
 
-template <int MODE>
+template <int MODE, bool SOFT>
 __device__ __forceinline__ typename SymV<MODE>::type sym_pair(
     double xi, double yi, double zi, double vxi, double vyi, double vzi, double mi, float mif,
     double xj, double yj, double zj, double vxj, double vyj, double vzj, double mj, double eta,
     double c_ff, double eps2) {
     if constexpr (MODE == HHS_TIMESTEP_FP32)
-        return pair_tau_f32(xi, yi, zi, vxi, vyi, vzi, mif, xj, yj, zj, vxj, vyj, vzj,
+        return pair_tau_f32<SOFT>(xi, yi, zi, vxi, vyi, vzi, mif, xj, yj, zj, vxj, vyj, vzj,
                             static_cast<float>(mj), static_cast<float>(eta),
                             static_cast<float>(c_ff), static_cast<float>(eps2));
     else
@@ -247,7 +247,7 @@ __device__ __forceinline__ typename SymV<MODE>::type sym_pair(
                                 c_ff, eps2);
 }
 
-template <int MODE>
+template <int MODE, bool SOFT>
 __global__ __launch_bounds__(256) void timesteps_sym_kernel(
     const double *__restrict__ pos, const double *__restrict__ vel, const double *__restrict__ m,
     int n, double eta, double eps2, typename SymV<MODE>::type *__restrict__ part) {
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
             const double mj = m[j];
 #pragma unroll
             for (int s = 0; s < kSymSpl; ++s) {
-                const V v = sym_pair<MODE>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
+                const V v = sym_pair<MODE, SOFT>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
                                            mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
                 row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
             }
@@ -300,10 +300,10 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
             V c = huge;
 #pragma unroll
             for (int s = 0; s < kSymSpl; ++s) {
-                const V v = sym_pair<MODE>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
+                const V v = sym_pair<MODE, SOFT>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
                                            mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
                 row[s] = vmin(row[s], v);
-                c = (ii[s] < n) ? vmin(c, v) : c;
+                c = vmin(c, v);  // I < J: block I is full, every ii[s] < n
             }
             c = wave_min(c);
             if (lane == 0) colmin[wave][j - j0] = c;
@@ -355,15 +355,19 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
     const dim3 grid(nb, nb), red((n + 255) / 256);
     if (mode == HHS_TIMESTEP_FP32) {
         float *part = static_cast<float *>(ws);
-        hipLaunchKernelGGL(timesteps_sym_kernel<HHS_TIMESTEP_FP32>, grid, dim3(256), 0, s, pos, vel,
-                           m, n, eta, eps2, part);
+        if (static_cast<float>(eps2) > 0.0f)
+            hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_FP32, true>), grid, dim3(256), 0,
+                               s, pos, vel, m, n, eta, eps2, part);
+        else
+            hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_FP32, false>), grid, dim3(256),
+                               0, s, pos, vel, m, n, eta, eps2, part);
         HHS_LAUNCH_CHECK("timesteps_sym_kernel");
         hipLaunchKernelGGL(timesteps_sym_reduce_kernel<float>, red, dim3(256), 0, s, part, nb, n,
                            ts);
     } else {
         double *part = static_cast<double *>(ws);
-        hipLaunchKernelGGL(timesteps_sym_kernel<HHS_TIMESTEP_EXACT>, grid, dim3(256), 0, s, pos,
-                           vel, m, n, eta, eps2, part);
+        hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_EXACT, false>), grid, dim3(256), 0,
+                           s, pos, vel, m, n, eta, eps2, part);
         HHS_LAUNCH_CHECK("timesteps_sym_kernel");
         hipLaunchKernelGGL(timesteps_sym_reduce_kernel<double>, red, dim3(256), 0, s, part, nb, n,
                            ts);
