@@ -10,9 +10,10 @@
 // * Work distribution: workgroups pull systems from two atomic queues --
 //   "big" systems (whole workgroup per system) first, then "small" systems
 //   (each wave pulls its own).  The host orders each queue largest-first.
-// * A system's numbers depend only on its own data and on its team size,
-//   which depends only on its particle count, so a system integrates to the
-//   same bits alone or in any mix.
+// * A system's numbers depend only on its own data, not on the team size
+//   (per-sink source order, 64-source kick blocks from the range start, a
+//   stable partition, energies from the first wave), so a system integrates
+//   to the same bits alone or in any mix, on a wave or a workgroup team.
 // * In-place hierarchy: a recursion level owns a contiguous index range
 //   [b, e) of the system's working copy; the split partitions it stably,
 //   fast first, through a scratch copy.  No scatter-back is needed; the
@@ -41,13 +42,13 @@ struct Frame {
     int calc, pad;
 };
 
-struct TeamShared {
+template <int TEAM> struct TeamShared {
     Frame stack[kMaxFrames];
     int level_calls[kMaxFrames];  // calls per recursion level (relative)
     int sp;
     int status;
-    int wave_tot[kWaves];
-    double red[kWG];
+    int wave_tot[TEAM / 64];
+    double red[64];
 };
 
 struct EnsArgs {
@@ -73,8 +74,8 @@ struct EnsArgs {
     int ts_mode;
 };
 
-template <int TEAM> struct Team;
-template <> struct Team<kWG> {
+// A workgroup team (TEAM = the block size) or a one-wave team (TEAM = 64).
+template <int TEAM> struct Team {
     int t;
     __device__ Team() : t(threadIdx.x) {}
     __device__ void sync() const { __syncthreads(); }
@@ -99,7 +100,7 @@ __device__ __forceinline__ int wave_sum(int x) {
 
 // Sum of one int per thread over the team; every thread gets the total.
 template <int TEAM>
-__device__ int team_sum(int x, TeamShared &sh, const Team<TEAM> &tm) {
+__device__ int team_sum(int x, TeamShared<TEAM> &sh, const Team<TEAM> &tm) {
     x = wave_sum(x);
     if constexpr (TEAM == 64) {
         return x;
@@ -108,7 +109,7 @@ __device__ int team_sum(int x, TeamShared &sh, const Team<TEAM> &tm) {
         tm.sync();
         int s = 0;
 #pragma unroll
-        for (int k = 0; k < kWaves; ++k) s += sh.wave_tot[k];
+        for (int k = 0; k < TEAM / 64; ++k) s += sh.wave_tot[k];
         tm.sync();
         return s;
     }
@@ -117,7 +118,7 @@ __device__ int team_sum(int x, TeamShared &sh, const Team<TEAM> &tm) {
 // Exclusive rank of p among the team's threads in thread order, and the
 // team total.
 template <int TEAM>
-__device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared &sh,
+__device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared<TEAM> &sh,
                                const Team<TEAM> &tm) {
     const int lane = tm.t & 63;
     const uint64_t mask = __ballot(p);
@@ -132,7 +133,7 @@ __device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared &sh,
         tm.sync();
         int before = 0, all = 0;
 #pragma unroll
-        for (int k = 0; k < kWaves; ++k) {
+        for (int k = 0; k < TEAM / 64; ++k) {
             const int c = sh.wave_tot[k];
             before += (k < w) ? c : 0;
             all += c;
@@ -302,7 +303,8 @@ __device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, double *ti
 // Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
 // the fast count after the lone-particle rule (src/split.cuh:321-324).
 template <int TEAM>
-__device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivot, TeamShared &sh,
+__device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivot,
+                              TeamShared<TEAM> &sh,
                               const Team<TEAM> &tm) {
     int c = 0;
     for (int i = o + b + tm.t; i < o + e; i += TEAM) c += (a.wts[i] <= pivot) ? 1 : 0;
@@ -427,35 +429,40 @@ __device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, doub
 }
 
 // E = sum 1/2 m v^2 - sum_{i<j} m_i m_j / sqrt(r^2 + eps2), FP64, fixed
-// reduction order.
+// reduction order.  The team's first wave evaluates it alone (lane l takes
+// particles l, l + 64, ...; the 64 lane sums are added in lane order), so
+// the value does not depend on the team size: twice per system, it is a
+// negligible share of the integration.
 template <int TEAM>
-__device__ double team_energy(const EnsArgs &a, int o, int n, TeamShared &sh,
+__device__ double team_energy(const EnsArgs &a, int o, int n, TeamShared<TEAM> &sh,
                               const Team<TEAM> &tm) {
 #pragma clang fp contract(off)
-    double acc = 0.0;
-    const int jb = uni(o), je = uni(o + n);
-    for (int i0 = jb; i0 < je; i0 += TEAM) {
-        const int i = i0 + tm.t;
-        const int ic = i < je ? i : je - 1;
-        const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
-        double s = 0.0;
-        for (int j = i0 + 1; j < je; ++j) {
-            const double dx = a.wpos[3 * j] - xi, dy = a.wpos[3 * j + 1] - yi,
-                         dz = a.wpos[3 * j + 2] - zi;
-            const double r2 = dx * dx + dy * dy + dz * dz + a.eps2;
-            const double t = a.wm[j] / sqrt(r2);
-            if (j > i && r2 != 0.0) s += t;
+    if (tm.t < 64) {
+        double acc = 0.0;
+        const int jb = uni(o), je = uni(o + n);
+        for (int i0 = jb; i0 < je; i0 += 64) {
+            const int i = i0 + tm.t;
+            const int ic = i < je ? i : je - 1;
+            const double xi = a.wpos[3 * ic], yi = a.wpos[3 * ic + 1], zi = a.wpos[3 * ic + 2];
+            double s = 0.0;
+            for (int j = i0 + 1; j < je; ++j) {
+                const double dx = a.wpos[3 * j] - xi, dy = a.wpos[3 * j + 1] - yi,
+                             dz = a.wpos[3 * j + 2] - zi;
+                const double r2 = dx * dx + dy * dy + dz * dz + a.eps2;
+                const double t = a.wm[j] / sqrt(r2);
+                if (j > i && r2 != 0.0) s += t;
+            }
+            if (i < je) {
+                const double vx = a.wvel[3 * i], vy = a.wvel[3 * i + 1], vz = a.wvel[3 * i + 2];
+                acc += 0.5 * a.wm[i] * (vx * vx + vy * vy + vz * vz) - a.wm[i] * s;
+            }
         }
-        if (i < je) {
-            const double vx = a.wvel[3 * i], vy = a.wvel[3 * i + 1], vz = a.wvel[3 * i + 2];
-            acc += 0.5 * a.wm[i] * (vx * vx + vy * vy + vz * vz) - a.wm[i] * s;
-        }
+        sh.red[tm.t] = acc;
     }
-    sh.red[tm.t] = acc;
     tm.sync();
     double tot = 0.0;
 #pragma unroll 1
-    for (int k = 0; k < TEAM; ++k) tot += sh.red[k];
+    for (int k = 0; k < 64; ++k) tot += sh.red[k];
     tm.sync();
     return tot;
 }
@@ -471,7 +478,7 @@ struct SysStats {
 // this call.  Returns HHS_OK or a status code (uniform).
 template <int TEAM, bool SOFT>
 __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int calc,
-                               TeamShared &sh, double *tile, const Team<TEAM> &tm,
+                               TeamShared<TEAM> &sh, double *tile, const Team<TEAM> &tm,
                                SysStats &st) {
     if (tm.t == 0) {
         sh.sp = 1;
@@ -543,7 +550,7 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
 }
 
 template <int TEAM, bool SOFT>
-__device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double *tile,
+__device__ void team_run_system(const EnsArgs &a, int s, TeamShared<TEAM> &sh, double *tile,
                                 const Team<TEAM> &tm) {
     const int o = uni(a.offsets[s]);
     const int n = uni(a.offsets[s + 1]) - o;
@@ -607,7 +614,8 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared &sh, double 
 
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
-    __shared__ TeamShared sh[kWaves];
+    __shared__ TeamShared<kWG> sh_wg;
+    __shared__ TeamShared<64> sh[kWaves];
     // Source tiles: the whole array for a workgroup team (kWG sources), a
     // quarter per wave team (64 sources each).
     __shared__ double tiles[kWG * kTsStride];
@@ -622,7 +630,7 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
         __syncthreads();
         if (k >= a.n_big) break;
         Team<kWG> tm;
-        team_run_system<kWG, SOFT>(a, a.big_list[k], sh[0], tiles, tm);
+        team_run_system<kWG, SOFT>(a, a.big_list[k], sh_wg, tiles, tm);
     }
     // Small systems: each wave pulls its own.
     const int w = threadIdx.x >> 6;
@@ -647,7 +655,7 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<double> total, int n,
                                                       double dt, int calc, SubtreeStats *out) {
-    __shared__ TeamShared sh;
+    __shared__ TeamShared<kWG> sh;
     __shared__ double tile[kWG * kTsStride];
     Team<kWG> tm;
     for (int i = tm.t; i < n; i += kWG) {
@@ -871,7 +879,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
             v > 0)
             cus = v;
     }
-    const long long items = static_cast<long long>(big.size()) + (small.size() + 3) / 4;
+    const long long items = static_cast<long long>(a.n_big) + (small.size() + 3) / 4;
     const int grid = static_cast<int>(std::max(1LL, std::min<long long>(items, 8LL * cus)));
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL(ensemble_kernel<true>, dim3(grid), dim3(kWG), 0, st, a);

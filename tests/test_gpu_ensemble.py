@@ -129,3 +129,22 @@ def test_ensemble_timestep_modes_track_oracle(hhs, oracle, torch_cuda, mode):
         assert res[k]["status"] == 0
         p, _ = ens.state(k)
         assert np.abs(p - host.pos).max() <= 1e-6 * np.sqrt((host.pos ** 2).mean())
+
+
+def test_ensemble_team_size_does_not_change_bits(hhs, torch_cuda):
+    """The same systems on one-wave teams (wave_team_max_n large) and on
+    256-thread workgroup teams (wave_team_max_n small): identical positions,
+    velocities, energies and statistics."""
+    eps, eta = 1e-4, 0.01
+    systems = _systems(hhs, [70, 300, 700, 1100], 1300, eps)
+    runs = []
+    for wave_max in (8, 4096):
+        ens = hhs.Ensemble(systems, 0.125, 1.0 / 32, eta, eps, wave_team_max_n=wave_max,
+                           fast_timesteps=hhs.TIMESTEP_FP32)
+        ens.run()
+        torch_cuda.cuda.synchronize()
+        runs.append((ens.results(), [ens.state(k) for k in range(len(systems))]))
+    (ra, sa), (rb, sb) = runs
+    assert ra.tobytes() == rb.tobytes()
+    for (pa, va), (pb, vb) in zip(sa, sb):
+        assert np.array_equal(pa, pb) and np.array_equal(va, vb)
