@@ -7,9 +7,10 @@
 // (root steps, the recursion of src/hold.cuh:51-181 as an explicit stack in
 // LDS, timesteps, splits, drifts, kicks) without returning to the host.
 //
-// * Work distribution: workgroups pull systems from two atomic queues --
-//   "big" systems (whole workgroup per system) first, then "small" systems
-//   (each wave pulls its own).  The host orders each queue largest-first.
+// * Work distribution: workgroups pull work items from one atomic queue,
+//   ordered on the device by a cost estimate from the initial state (largest
+//   first): a "big" system takes the whole workgroup, a bundle of four
+//   "small" systems one wave each.
 // * A system's numbers depend only on its own data, not on the team size
 //   (per-sink source order, 64-source kick blocks from the range start, a
 //   stable partition, energies from the first wave), so a system integrates
@@ -35,6 +36,7 @@ constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
 constexpr int kMaxFrames = kSubtreeLevels;  // max_level <= 47 plus root and one child
 constexpr int kFold = 64;
+constexpr int kCounters = 16;  // queue counters in the workspace
 
 struct Frame {
     int b, e, nf, stage;
@@ -68,6 +70,11 @@ struct EnsArgs {
     int n_big;
     const int *small_list;
     int n_small;
+    double *est;        // cost estimate per system
+    int *small_sorted;  // small systems, largest estimate first
+    int *queue;         // work items, largest estimate first: >= 0 a big
+                        // system, < 0 a bundle of four small ones
+    int n_items;
     unsigned int *counters;
     hhs_system_result *results;
     int *done;
@@ -612,6 +619,96 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared<TEAM> &sh, d
     tm.sync();
 }
 
+// ------------------------------------------------------------ scheduling
+// A system's cost is not a function of its size alone: one tight binary
+// makes a 55-particle system recurse deep enough to cost as much as a
+// 1500-particle one.  So the batch is ordered by an estimate from the
+// initial state, largest first (list scheduling), and teams pull work items
+// in that order: a big system on the whole workgroup, or a bundle of four
+// small systems with neighbouring estimates, one per wave.
+//
+// Estimate: the initial timesteps give each particle the level it descends
+// to (tau <= dt_root / 2^(L-1) reaches level L); with n_L particles at
+// level L or deeper, a root step makes about 2^L calls on n_L particles,
+// each costing ~n_L^2 pair evaluations plus a fixed overhead.
+constexpr double kCallCost = 16384.0;  // per-call overhead, in pair evaluations
+constexpr int kEstSinks = 1024;        // sinks sampled per system
+
+__global__ __launch_bounds__(64) void ensemble_estimate_kernel(EnsArgs a) {
+    __shared__ int hist[kMaxFrames];
+    const int s = blockIdx.x, lane = threadIdx.x;
+    const int o = a.offsets[s], n = a.offsets[s + 1] - o;
+    for (int l = lane; l < kMaxFrames; l += 64) hist[l] = 0;
+    __syncthreads();
+    const double dtr = a.dt_root[s], t_end = a.t_end[s];
+    const int lmax = min(a.max_level, kMaxFrames - 1);
+    if (n >= 2 && dtr > 0.0) {
+        const float etaf = static_cast<float>(a.eta), cff = static_cast<float>(a.c_ff);
+        // At most kEstSinks sinks (every stride-th particle), each counted
+        // stride times: O(n) work per sink keeps large systems cheap.
+        const int stride = (n + kEstSinks - 1) / kEstSinks;
+        const int ns = (n + stride - 1) / stride;
+        for (int q0 = 0; q0 < ns; q0 += 64) {
+            const int q = q0 + lane;
+            const int i = (q < ns ? q : ns - 1) * stride;
+            const int ic = o + i;
+            const double xi = a.pos[3 * ic], yi = a.pos[3 * ic + 1], zi = a.pos[3 * ic + 2];
+            const double vxi = a.vel[3 * ic], vyi = a.vel[3 * ic + 1], vzi = a.vel[3 * ic + 2];
+            const float mi = static_cast<float>(a.m[ic]);
+            float bnum = static_cast<float>(TsConst<double>::huge), bden = 1.0f;
+            for (int j = o; j < o + n; ++j)
+                pair_timestep_f32(xi, yi, zi, vxi, vyi, vzi, mi, a.pos[3 * j], a.pos[3 * j + 1],
+                                  a.pos[3 * j + 2], a.vel[3 * j], a.vel[3 * j + 1],
+                                  a.vel[3 * j + 2], static_cast<float>(a.m[j]), etaf, cff,
+                                  a.eps2f, j != ic, bnum, bden);
+            if (q < ns) {
+                const double tau = static_cast<double>(bnum) / static_cast<double>(bden);
+                int l = 0;
+                for (double dt = dtr; l < lmax && tau <= dt; dt *= 0.5) ++l;
+                atomicAdd(&hist[l], stride);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double work = 0.0;
+        int nl = 0;
+        for (int l = kMaxFrames - 1; l >= 0; --l) {
+            nl += hist[l];
+            if (nl >= 2) work += ldexp(1.0, l) * (static_cast<double>(nl) * nl + kCallCost);
+        }
+        const double steps = (dtr > 0.0 && t_end > 0.0) ? ceil(t_end / dtr) : 0.0;
+        a.est[s] = work * steps + n;
+    }
+}
+
+// Rank of item k among all items by (estimate descending, index ascending).
+__device__ __forceinline__ bool before(double ej, int j, double ek, int k) {
+    return ej > ek || (ej == ek && j < k);
+}
+
+__global__ __launch_bounds__(256) void rank_small_kernel(EnsArgs a) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= a.n_small) return;
+    const double ek = a.est[a.small_list[k]];
+    int r = 0;
+    for (int j = 0; j < a.n_small; ++j) r += before(a.est[a.small_list[j]], j, ek, k) ? 1 : 0;
+    a.small_sorted[r] = a.small_list[k];
+}
+
+__device__ __forceinline__ double item_est(const EnsArgs &a, int k) {
+    return k < a.n_big ? a.est[a.big_list[k]] : a.est[a.small_sorted[4 * (k - a.n_big)]];
+}
+
+__global__ __launch_bounds__(256) void rank_items_kernel(EnsArgs a) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= a.n_items) return;
+    const double ek = item_est(a, k);
+    int r = 0;
+    for (int j = 0; j < a.n_items; ++j) r += before(item_est(a, j), j, ek, k) ? 1 : 0;
+    a.queue[r] = k < a.n_big ? a.big_list[k] : -1 - (k - a.n_big);
+}
+
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
     __shared__ TeamShared<kWG> sh_wg;
@@ -620,7 +717,7 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
     // quarter per wave team (64 sources each).
     __shared__ double tiles[kWG * kTsStride];
     __shared__ int s_item;
-    // Big systems: the whole workgroup on one system.
+    const int w = threadIdx.x >> 6;
     for (;;) {
         if (threadIdx.x == 0)
             s_item = static_cast<int>(__hip_atomic_fetch_add(&a.counters[0], 1u, __ATOMIC_RELAXED,
@@ -628,30 +725,23 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
         __syncthreads();
         const int k = s_item;
         __syncthreads();
-        if (k >= a.n_big) break;
-        Team<kWG> tm;
-        team_run_system<kWG, SOFT>(a, a.big_list[k], sh_wg, tiles, tm);
-    }
-    // Small systems: each wave pulls its own.
-    const int w = threadIdx.x >> 6;
-    Team<64> tw;
-    for (;;) {
-        int k = 0;
-        if (tw.t == 0)
-            k = static_cast<int>(__hip_atomic_fetch_add(&a.counters[1], 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT));
-        k = __shfl(k, 0, 64);
-        if (k >= a.n_small) break;
-        team_run_system<64, SOFT>(a, a.small_list[k], sh[w], tiles + w * 64 * kTsStride, tw);
+        if (k >= a.n_items) break;
+        const int code = a.queue[k];
+        if (code >= 0) {
+            Team<kWG> tm;
+            team_run_system<kWG, SOFT>(a, code, sh_wg, tiles, tm);
+        } else {
+            const int idx = 4 * (-1 - code) + w;
+            if (idx < a.n_small) {
+                Team<64> tw;
+                team_run_system<64, SOFT>(a, a.small_sorted[idx], sh[w],
+                                          tiles + w * 64 * kTsStride, tw);
+            }
+            __syncthreads();  // the bundle is done before the tiles change hands
+        }
     }
 }
 
-// One recursion subtree of a per-system HOLD, run by one workgroup: the
-// particles of the call are copied into the working arrays (original index
-// in wid), the engine above integrates the subtree in place, and the
-// records (pos, vel, m, timestep) go back to their original slots.  The
-// host pays one launch and one sync for the whole subtree instead of one
-// sync and ~8 launches per call.
 template <bool SOFT>
 __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<double> total, int n,
                                                       double dt, int calc, SubtreeStats *out) {
@@ -693,13 +783,13 @@ __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<doubl
 }
 
 __global__ void zero_counters_kernel(unsigned int *c) {
-    if (threadIdx.x < 16) c[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < kCounters; i += blockDim.x) c[i] = 0u;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 struct EnsLayout {
-    size_t counters, big, small, wpos, wvel, wm, wts, wid, tpos, tvel, tm, tts, tid, total;
+    size_t counters, big, small, est, small_sorted, queue, wpos, wvel, wm, wts, wid, tpos, tvel, tm, tts, tid, total;
 };
 
 EnsLayout layout(int nsys, long long np) {
@@ -712,9 +802,12 @@ EnsLayout layout(int nsys, long long np) {
     };
     const size_t n = static_cast<size_t>(np > 0 ? np : 1);
     const size_t s = static_cast<size_t>(nsys > 0 ? nsys : 1);
-    L.counters = take(64);
+    L.counters = take(4 * kCounters);
     L.big = take(s * 4);
     L.small = take(s * 4);
+    L.est = take(s * 8);
+    L.small_sorted = take(s * 4);
+    L.queue = take(s * 4);
     L.wpos = take(3 * n * 8);
     L.wvel = take(3 * n * 8);
     L.wm = take(n * 8);
@@ -809,17 +902,11 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     // the lane count, bounds a big system.
     const int wave_max = d->wave_team_max_n > 0 ? d->wave_team_max_n : 512;
 
-    // Largest-first queues: whole-workgroup systems and one-wave systems.
+    // Whole-workgroup systems and one-wave systems, in index order (the
+    // device orders the work by estimated cost).
     std::vector<int> big, small;
     for (int s = 0; s < nsys; ++s)
         (offsets_host[s + 1] - offsets_host[s] > wave_max ? big : small).push_back(s);
-    auto by_size = [&](int x, int y) {
-        const int nx = offsets_host[x + 1] - offsets_host[x];
-        const int ny = offsets_host[y + 1] - offsets_host[y];
-        return nx != ny ? nx > ny : x < y;
-    };
-    std::sort(big.begin(), big.end(), by_size);
-    std::sort(small.begin(), small.end(), by_size);
 
     char *w = static_cast<char *>(ws);
     hipStream_t st = as_stream(stream);
@@ -832,7 +919,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
         HHS_HIP(hipMemcpyAsync(w + L.small, small.data(), small.size() * 4, hipMemcpyHostToDevice,
                                st));
     unsigned int *counters = reinterpret_cast<unsigned int *>(w + L.counters);
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, counters);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(256), 0, st, counters);
     HHS_LAUNCH_CHECK("zero_counters_kernel");
 
     EnsArgs a;
@@ -862,6 +949,10 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     a.n_big = static_cast<int>(big.size());
     a.small_list = reinterpret_cast<const int *>(w + L.small);
     a.n_small = static_cast<int>(small.size());
+    a.est = reinterpret_cast<double *>(w + L.est);
+    a.small_sorted = reinterpret_cast<int *>(w + L.small_sorted);
+    a.queue = reinterpret_cast<int *>(w + L.queue);
+    a.n_items = a.n_big + (a.n_small + 3) / 4;
     a.counters = counters;
     a.results = d->results;
     a.done = d->done;
@@ -879,8 +970,16 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
             v > 0)
             cus = v;
     }
-    const long long items = static_cast<long long>(a.n_big) + (small.size() + 3) / 4;
-    const int grid = static_cast<int>(std::max(1LL, std::min<long long>(items, 8LL * cus)));
+    // Cost estimates, then the work queue in estimate order.
+    hipLaunchKernelGGL(ensemble_estimate_kernel, dim3(nsys), dim3(64), 0, st, a);
+    HHS_LAUNCH_CHECK("ensemble_estimate_kernel");
+    if (a.n_small > 0) {
+        hipLaunchKernelGGL(rank_small_kernel, dim3((a.n_small + 255) / 256), dim3(256), 0, st, a);
+        HHS_LAUNCH_CHECK("rank_small_kernel");
+    }
+    hipLaunchKernelGGL(rank_items_kernel, dim3((a.n_items + 255) / 256), dim3(256), 0, st, a);
+    HHS_LAUNCH_CHECK("rank_items_kernel");
+    const int grid = std::max(1, std::min(a.n_items, 8 * cus));
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL(ensemble_kernel<true>, dim3(grid), dim3(kWG), 0, st, a);
     else
