@@ -742,11 +742,35 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
     }
 }
 
-template <bool SOFT>
+// Subtrees of at most kSubLds particles keep their working and scratch
+// copies in LDS (2 x 68 B per particle, 34 KB at 256): the subtree is the
+// only work on the GPU while it runs, so each of a call's dependent memory
+// round trips (tile loads, partition, drifts, velocity updates) costs LDS
+// latency instead of L2 latency.  The engine reaches them through generic
+// pointers; the arithmetic and its order are those of the global-memory
+// variant, so both give the same bits.
+constexpr int kSubLds = 256;
+
+template <bool SOFT, bool LDS>
 __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<double> total, int n,
                                                       double dt, int calc, SubtreeStats *out) {
     __shared__ TeamShared<kWG> sh;
     __shared__ double tile[kWG * kTsStride];
+    constexpr int C = LDS ? kSubLds : 1;
+    __shared__ double l_pos[2][3 * C], l_vel[2][3 * C], l_m[2][C], l_ts[2][C];
+    __shared__ int l_id[2][C];
+    if constexpr (LDS) {
+        a.wpos = l_pos[0];
+        a.wvel = l_vel[0];
+        a.wm = l_m[0];
+        a.wts = l_ts[0];
+        a.wid = l_id[0];
+        a.tpos = l_pos[1];
+        a.tvel = l_vel[1];
+        a.tm = l_m[1];
+        a.tts = l_ts[1];
+        a.tid = l_id[1];
+    }
     Team<kWG> tm;
     for (int i = tm.t; i < n; i += kWG) {
 #pragma unroll
@@ -856,12 +880,12 @@ int launch_subtree(const Particles<double> &total, int n, double dt, int calc, d
     a.tts = reinterpret_cast<double *>(w + L.tts);
     a.tid = reinterpret_cast<int *>(w + L.tid);
     a.ts_mode = ts_mode;
-    if (eps2 > 0.0)
-        hipLaunchKernelGGL(subtree_kernel<true>, dim3(1), dim3(kWG), 0, st, a, total, n, dt, calc,
-                           out_dev);
-    else
-        hipLaunchKernelGGL(subtree_kernel<false>, dim3(1), dim3(kWG), 0, st, a, total, n, dt,
-                           calc, out_dev);
+    const bool lds = n <= kSubLds;
+    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula
+    auto kern = static_cast<float>(eps2) > 0.0f
+                    ? (lds ? subtree_kernel<true, true> : subtree_kernel<true, false>)
+                    : (lds ? subtree_kernel<false, true> : subtree_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kWG), 0, st, a, total, n, dt, calc, out_dev);
     HHS_LAUNCH_CHECK("subtree_kernel");
     return HHS_OK;
 }
@@ -980,7 +1004,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     hipLaunchKernelGGL(rank_items_kernel, dim3((a.n_items + 255) / 256), dim3(256), 0, st, a);
     HHS_LAUNCH_CHECK("rank_items_kernel");
     const int grid = std::max(1, std::min(a.n_items, 8 * cus));
-    if (a.eps2 > 0.0)
+    if (a.eps2f > 0.0f)  // eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula
         hipLaunchKernelGGL(ensemble_kernel<true>, dim3(grid), dim3(kWG), 0, st, a);
     else
         hipLaunchKernelGGL(ensemble_kernel<false>, dim3(grid), dim3(kWG), 0, st, a);

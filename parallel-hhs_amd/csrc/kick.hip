@@ -407,11 +407,12 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
 // its latency.  MI355X, square kicks: N = 4096 48 -> 21 us, 8192 96 ->
 // 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.
 constexpr size_t kMaxPartialBytes = size_t{4} << 20;
+constexpr int kSplitMinSources = 1024;
 
 int pick_split(int n_sink, int n_src, int S, int W) {
     const long long tiles = (n_sink + 64LL * S - 1) / (64LL * S);
     int K = 1;
-    if (n_src < 2048) return K;  // the reduce launch would cost more than it saves
+    if (n_src < kSplitMinSources) return K;  // the reduce launch would cost more than it saves
     while (K < 64 && tiles * W * K * 2 <= 8192 && n_src / (W * K * 2) >= 32 &&
            24ULL * n_sink * K * 2 <= kMaxPartialBytes)
         K *= 2;

@@ -130,14 +130,16 @@ int pick_waves(int n) {
 }
 
 // Split-K as for the kick: more workgroups over chunks of the sources
-// while the grid stays under 8192 waves, each wave keeps >= 32 sources and
-// the partials (two values per sink and chunk) stay within 4 MB.  MI355X,
-// FP32 mode: n = 4096 143 -> 33 us, 8192 283 -> 94 us, 16384 577 -> 340 us.
+// while the grid stays under 8192 waves, each wave keeps >= kTsMinChunk
+// sources and the partials (two values per sink and chunk) stay within
+// 4 MB.  MI355X, FP32 mode: n = 4096 143 -> 33 us, 8192 283 -> 94 us, 16384
+// 577 -> 340 us.
+constexpr int kTsMinN = 256, kTsMinChunk = 16;
 template <typename V> int pick_ts_split(int n, int W) {
     const long long tiles = (n + 63) / 64;
     int K = 1;
-    if (n < 2048) return K;
-    while (K < 64 && tiles * W * K * 2 <= 8192 && n / (W * K * 2) >= 32 &&
+    if (n < kTsMinN) return K;
+    while (K < 64 && tiles * W * K * 2 <= 8192 && n / (W * K * 2) >= kTsMinChunk &&
            2ULL * sizeof(V) * n * K * 2 <= (size_t{4} << 20))
         K *= 2;
     return K;

@@ -162,6 +162,51 @@ def test_hold_depth_guard(hhs, torch_cuda, subtree_max):
     assert ei.value.code == hhs.HHS_ERR_DEPTH
 
 
+@pytest.mark.parametrize("n,eps", [(2, 1e-3), (5, 0.0), (64, 1e-3), (65, 0.0), (200, 1e-3),
+                                   (256, 1e-4), (300, 1e-3), (700, 0.0), (1100, 1e-4),
+                                   (2500, 1e-3)])
+def test_subtree_fp32_timesteps_match_oracle(hhs, oracle, torch_cuda, n, eps):
+    """FP32 timesteps of the device engine (each unordered pair once, column
+    minima by atomics) against the FP64 oracle, within 1e-6 relative: a root
+    step whose dt is below every timestep runs one call (all particles
+    slow), so the timesteps left behind are those of the root call.  Up to
+    256 particles the particles sit in LDS and each lane holds one sink;
+    above, global memory and several sinks per lane and passes."""
+    eta = 0.01
+    pos, vel, m = plummer(hhs, n, seed=6060 + n, eps2=eps * eps)
+    ref = oracle.timesteps(pos, vel, m, eta, eps * eps)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(n, eta, eps, max_level=40, subtree_max=4096,
+                    fast_timesteps=hhs.TIMESTEP_FP32)
+    g.hold_step(d, float(ref.min()) * 0.5)
+    torch_cuda.cuda.synchronize()
+    assert g.stats()["calls"] == 1
+    ts = d.timestep.cpu().numpy()
+    assert np.all(np.abs(ts - ref) <= 1e-6 * ref)
+
+
+@pytest.mark.parametrize("n", [37, 256])
+def test_hold_lds_subtree_bits_equal_ensemble(hhs, torch_cuda, n):
+    """A root step run as one device subtree (n <= 256: the particles live in
+    LDS) gives the same bits as the ensemble engine on the same system
+    (particles in global memory, a one-wave team): the arithmetic and its
+    order do not depend on where the particles are held."""
+    eps, eta, dt = 1e-3, 0.01, 1.0 / 16
+    pos, vel, m = plummer(hhs, n, seed=5150 + n, eps2=eps * eps)
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    # max_level 40 leaves the root call itself inside the subtree's depth range
+    g = hhs.Globals(n, eta, eps, max_level=40, subtree_max=256)
+    g.hold(d, dt, dt)
+    ens = hhs.Ensemble([(pos, vel, m)], dt, dt, eta, eps, max_level=40)
+    ens.run()
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert st["calls"] == ens.results()[0]["calls"] and st["calls"] > 1
+    p, v = ens.state(0)
+    assert d.pos.cpu().numpy().tobytes() == p.tobytes()
+    assert d.vel.cpu().numpy().tobytes() == v.tobytes()
+
+
 @pytest.mark.parametrize("subtree_max", [32, 256])
 def test_hold_subtrees_several_root_steps(hhs, oracle, torch_cuda, subtree_max):
     """Device subtrees inside a longer run: same recursion and simtime as
