@@ -133,6 +133,19 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
 
 // ------------------------------------------------------------ drift/update
 template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStream_t s);
+// HOLD-internal fused forms (hold.hip): the split that also half-drifts the
+// slow set and reports the raw fast count, its lone-particle follow-up, and
+// the scatter-back of both sets with the second slow half-drift.
+template <typename T>
+int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
+                      const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
+                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st);
+template <typename T>
+int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
+                           hipStream_t st);
+template <typename T>
+int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int nf,
+                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s);
 template <typename T>
 int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
 

@@ -12,8 +12,13 @@
 //    source segments (slow, fast), so each slow particle's force is summed
 //    once (src/hold.cuh:137-145); the exact FP64 kick mode keeps the two
 //    launches so its sums follow the CPU order;
-//  * the only host synchronisation per call is the 8-byte count read after
-//    the split (the reference blocks at the same point, src/split.cuh:430);
+//  * the only host synchronisation per call is the count read after the
+//    split (the reference blocks at the same point, src/split.cuh:430); the
+//    split writes the counts straight into host-mapped memory;
+//  * the two slow half-drifts ride on other launches: the first on the
+//    split's scatter of the slow set, the second on the scatter-back (the
+//    fast recursions in between never read the slow set), and both sets
+//    are scattered back by one launch;
 //  * a max_level guard returns HHS_ERR_DEPTH instead of recursing without
 //    bound when timesteps collapse (two coincident particles with eps = 0).
 #include <cstring>
@@ -39,8 +44,10 @@ struct hhs_globals {
     int *scanned = nullptr;
     void *scan_ws = nullptr;
     size_t scan_ws_bytes = 0;
-    hhs_partition *part_dev = nullptr;   // one slot per level
-    hhs_partition *part_host = nullptr;  // pinned mirror
+    // per level: counts and the raw fast count, in host-mapped pinned memory
+    void *part_block = nullptr;
+    hhs_partition *part_host = nullptr, *part_map = nullptr;  // host / device address
+    int *raw_host = nullptr, *raw_map = nullptr;
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
@@ -189,23 +196,26 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     }
     if ((rc = ensure_level(g, clevel))) return rc;
     const LevelBuffers lb = g->levels[clevel];  // by value: plain pointers
-    // src/hold.cuh:79-83, then the count read-back (src/split.cuh:430)
-    rc = launch_split<double>(total, lb.slow, lb.fast, nullptr, nullptr, dt, n,
-                              g->part_dev + clevel, g->scan_ws, g->scan_ws_bytes, st);
+    const double h = dt * 0.5;  // the half-drift of src/hold.cuh:123-124,158-159
+    // src/hold.cuh:79-83 with the first slow half-drift, then the count
+    // read-back (src/split.cuh:430)
+    rc = launch_split_hold<double>(total, lb.slow, lb.fast, dt, n, g->part_map + clevel,
+                                   g->raw_map + clevel, h, g->scan_ws, g->scan_ws_bytes, st);
     if (rc) return rc;
-    HHS_HIP(hipMemcpyAsync(g->part_host + clevel, g->part_dev + clevel, sizeof(hhs_partition),
-                           hipMemcpyDeviceToHost, st));
     HHS_HIP(hipStreamSynchronize(st));
     const int ns = g->part_host[clevel].N_slow;
     const int nf = g->part_host[clevel].N_fast;
+    // a lone fast particle stays slow (src/split.cuh:321-324): the slow set
+    // becomes the whole set, drifted
+    if (g->raw_host[clevel] == 1 && (rc = launch_lone_fast_fixup<double>(total, lb.slow, n, h, st)))
+        return rc;
 
     // src/hold.cuh:106-114
     if (nf == 0) g->simtime += dt;
     // src/hold.cuh:116-121
     if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, false, st))) return rc;
     if (ns > 0) {
-        // src/hold.cuh:123-124
-        if ((rc = launch_drift<double>(lb.slow.pos, lb.slow.vel, ns, dt * 0.5, st))) return rc;
+        // src/hold.cuh:123-124: done by the split
         // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow.
         // In the exact FP64 mode the S<-S and S<-F kicks are two velocity
         // updates, as in the reference, so the sums match the CPU order.
@@ -229,15 +239,13 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
             if ((rc = launch_kick<double>(kf, st))) return rc;
             S.interactions += static_cast<long long>(nf) * ns;
         }
-        // src/hold.cuh:158-159
-        if ((rc = launch_drift<double>(lb.slow.pos, lb.slow.vel, ns, dt * 0.5, st))) return rc;
+        // src/hold.cuh:158-159: done by the scatter-back below
     }
     // src/hold.cuh:168-173
     if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, true, st))) return rc;
-    // src/hold.cuh:177-178 (the deeper levels reuse lb, so scatter back now)
-    if ((rc = launch_update<double>(total, lb.fast, nf, st))) return rc;
-    if ((rc = launch_update<double>(total, lb.slow, ns, st))) return rc;
-    return HHS_OK;
+    // src/hold.cuh:177-178 (the deeper levels reuse lb, so scatter back now),
+    // with the second slow half-drift
+    return launch_update_pair<double>(total, lb.fast, nf, lb.slow, ns, h, st);
 }
 
 }  // namespace
@@ -270,10 +278,18 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
     hipError_t e = hipMalloc(&g->predicate, sizeof(int) * static_cast<size_t>(n_total));
     if (e == hipSuccess) e = hipMalloc(&g->scanned, sizeof(int) * static_cast<size_t>(n_total));
     if (e == hipSuccess) e = hipMalloc(&g->scan_ws, g->scan_ws_bytes);
-    if (e == hipSuccess) e = hipMalloc(&g->part_dev, sizeof(hhs_partition) * (max_level + 1));
+    const size_t levels = static_cast<size_t>(max_level) + 1;
     if (e == hipSuccess)
-        e = hipHostMalloc(&g->part_host, sizeof(hhs_partition) * (max_level + 1),
-                          hipHostMallocDefault);
+        e = hipHostMalloc(&g->part_block, (sizeof(hhs_partition) + sizeof(int)) * levels,
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    void *map = nullptr;
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&map, g->part_block, 0);
+    if (e == hipSuccess) {
+        g->part_host = static_cast<hhs_partition *>(g->part_block);
+        g->part_map = static_cast<hhs_partition *>(map);
+        g->raw_host = reinterpret_cast<int *>(g->part_host + levels);
+        g->raw_map = reinterpret_cast<int *>(g->part_map + levels);
+    }
     if (e == hipSuccess) e = hipMemset(g->scan_ws, 0, g->scan_ws_bytes);
     if (e != hipSuccess) {
         int rc = hip_status(e, "globals_create", __FILE__, __LINE__);
@@ -306,8 +322,7 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->predicate) (void)hipFree(g->predicate);
     if (g->scanned) (void)hipFree(g->scanned);
     if (g->scan_ws) (void)hipFree(g->scan_ws);
-    if (g->part_dev) (void)hipFree(g->part_dev);
-    if (g->part_host) (void)hipHostFree(g->part_host);
+    if (g->part_block) (void)hipHostFree(g->part_block);
     if (g->sub_ws) (void)hipFree(g->sub_ws);
     if (g->ts_ws) (void)hipFree(g->ts_ws);
     if (g->sub_dev) (void)hipFree(g->sub_dev);

@@ -177,6 +177,11 @@ template <typename T> struct SplitOp {
     T pivot;
     int n;
     hhs_partition *part;
+    // HOLD only: drift the slow records as they are written (pos + vel * h,
+    // the drift kernel's expression) and report the raw fast count.
+    int drift;
+    T h;
+    int *raw;
     using Item = SplitRec<T>;
     // src/split.cuh:115 -- NaN compares false and counts as slow.
     __device__ int value_at(int e) const { return total.timestep[e] <= pivot ? 1 : 0; }
@@ -195,13 +200,15 @@ template <typename T> struct SplitOp {
     }
     __device__ int value(const Item &r) const { return r.ts <= pivot ? 1 : 0; }
     __device__ void emit(int e, const Item &r, int p, int f) const {
+#pragma clang fp contract(off)
         if (predicate) predicate[e] = p;
         if (scanned) scanned[e] = f;
         const Particles<T> &d = p ? fast : slow;
         const int o = p ? f : e - f;  // src/split.cuh:336,350
-        d.pos[3 * o + 0] = r.px;
-        d.pos[3 * o + 1] = r.py;
-        d.pos[3 * o + 2] = r.pz;
+        const bool dr = drift && !p;
+        d.pos[3 * o + 0] = dr ? r.px + r.vx * h : r.px;
+        d.pos[3 * o + 1] = dr ? r.py + r.vy * h : r.py;
+        d.pos[3 * o + 2] = dr ? r.pz + r.vz * h : r.pz;
         d.vel[3 * o + 0] = r.vx;
         d.vel[3 * o + 1] = r.vy;
         d.vel[3 * o + 2] = r.vz;
@@ -215,6 +222,7 @@ template <typename T> struct SplitOp {
         const int reported = (nf == 1) ? 0 : nf;  // src/split.cuh:321-324
         part->N_fast = reported;
         part->N_slow = n - reported;
+        if (raw) *raw = nf;
     }
 };
 
@@ -243,15 +251,21 @@ template <typename T> struct SplitValuesOp {
 
 // Lone fast particle: the whole set stays slow, in order (the intended
 // behaviour of src/split.cuh:321-324; the reference's :361-368 races).
+// header null: unconditional (HOLD, which reads the raw count itself);
+// drift: positions written as pos + vel * h, as the HOLD split does.
 template <typename T>
 __global__ __launch_bounds__(256) void lone_fast_fixup_kernel(Particles<T> total,
                                                               Particles<T> slow, int n,
-                                                              const uint32_t *header) {
-    if (header[2] != 1u) return;
+                                                              const uint32_t *header, int drift,
+                                                              T h) {
+#pragma clang fp contract(off)
+    if (header && header[2] != 1u) return;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        slow.pos[3 * e + 0] = total.pos[3 * e + 0];
-        slow.pos[3 * e + 1] = total.pos[3 * e + 1];
-        slow.pos[3 * e + 2] = total.pos[3 * e + 2];
+        const T x = total.pos[3 * e + 0], y = total.pos[3 * e + 1], z = total.pos[3 * e + 2];
+        const T vx = total.vel[3 * e + 0], vy = total.vel[3 * e + 1], vz = total.vel[3 * e + 2];
+        slow.pos[3 * e + 0] = drift ? x + vx * h : x;
+        slow.pos[3 * e + 1] = drift ? y + vy * h : y;
+        slow.pos[3 * e + 2] = drift ? z + vz * h : z;
         slow.vel[3 * e + 0] = total.vel[3 * e + 0];
         slow.vel[3 * e + 1] = total.vel[3 * e + 1];
         slow.vel[3 * e + 2] = total.vel[3 * e + 2];
@@ -346,10 +360,41 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
     if (rc) return rc;
     const int fix_groups = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
     hipLaunchKernelGGL((lone_fast_fixup_kernel<T>), dim3(fix_groups), dim3(256), 0, st, total,
-                       slow, n, static_cast<const uint32_t *>(ws));
+                       slow, n, static_cast<const uint32_t *>(ws), 0, T(0));
     HHS_LAUNCH_CHECK("lone_fast_fixup_kernel");
     return HHS_OK;
 }
+
+// The HOLD form of the split (one call of src/hold.cuh:79-83 plus the
+// first slow half-drift of :123-124, which the fast recursion does not
+// read): slow records are written drifted by h_slow, the counts go to
+// `part` and the raw fast count to `raw` (host-mapped memory, read after a
+// stream sync), and no lone-particle follow-up is launched -- the caller
+// runs launch_lone_fast_fixup when *raw == 1.
+template <typename T>
+int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
+                      const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
+                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st) {
+    HHS_REQUIRE(n > 0 && part && raw, "split_hold: bad arguments");
+    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, raw};
+    return launch_scan_op<true>(op, n, ws, ws_bytes, st);
+}
+
+template <typename T>
+int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
+                           hipStream_t st) {
+    const int fix_groups = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
+    hipLaunchKernelGGL((lone_fast_fixup_kernel<T>), dim3(fix_groups), dim3(256), 0, st, total,
+                       slow, n, static_cast<const uint32_t *>(nullptr), 1, h_slow);
+    HHS_LAUNCH_CHECK("lone_fast_fixup_kernel");
+    return HHS_OK;
+}
+
+template int launch_split_hold<double>(const Particles<double> &, const Particles<double> &,
+                                       const Particles<double> &, double, int, hhs_partition *,
+                                       int *, double, void *, size_t, hipStream_t);
+template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
+                                            int, double, hipStream_t);
 
 template int launch_split<double>(const Particles<double> &, const Particles<double> &,
                                   const Particles<double> &, int *, int *, double, int,

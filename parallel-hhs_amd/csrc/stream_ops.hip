@@ -42,6 +42,42 @@ __global__ __launch_bounds__(256) void update_kernel(Particles<T> prev, Particle
     prev.timestep[p] = ts;
 }
 
+// HOLD's scatter-back of one call (src/hold.cuh:177-178) with the second
+// slow half-drift (:158-159) folded in: threads [0, nf) write fast record i
+// to prev[fast.parent_id[i]]; threads [nf, nf + ns) write slow record
+// i - nf, its position drifted as pos + vel * h (the drift kernel's
+// expression).  The fast recursion that runs between the drift and the
+// scatter-back never reads the slow set, so deferring the drift to here
+// changes no value.
+template <typename T>
+__global__ __launch_bounds__(256) void update_pair_kernel(Particles<T> prev, Particles<T> fast,
+                                                          int nf, Particles<T> slow, int ns,
+                                                          T h) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf + ns) return;
+    const bool is_fast = t < nf;
+    const Particles<T> &c = is_fast ? fast : slow;
+    const int i = is_fast ? t : t - nf;
+    const int p = c.parent_id[i];
+    T x = c.pos[3 * i + 0], y = c.pos[3 * i + 1], z = c.pos[3 * i + 2];
+    const T vx = c.vel[3 * i + 0], vy = c.vel[3 * i + 1], vz = c.vel[3 * i + 2];
+    const T m = c.m[i], ts = c.timestep[i];
+    if (!is_fast) {
+        x = x + vx * h;
+        y = y + vy * h;
+        z = z + vz * h;
+    }
+    prev.pos[3 * p + 0] = x;
+    prev.pos[3 * p + 1] = y;
+    prev.pos[3 * p + 2] = z;
+    prev.vel[3 * p + 0] = vx;
+    prev.vel[3 * p + 1] = vy;
+    prev.vel[3 * p + 2] = vz;
+    prev.m[p] = m;
+    prev.timestep[p] = ts;
+}
+
 // Row sums of the potential: row[i] = m_i * sum_{j>i} m_j / sqrt(r^2+eps2),
 // j ascending (the order of a serial CPU loop), plus the kinetic term of i.
 // The j loop is wave-uniform (scalar loads); lanes with j <= i skip.
@@ -133,6 +169,21 @@ template int launch_update<double>(const Particles<double> &, const Particles<do
                                    hipStream_t);
 template int launch_update<float>(const Particles<float> &, const Particles<float> &, int,
                                   hipStream_t);
+
+template <typename T>
+int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int nf,
+                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s) {
+    HHS_REQUIRE(nf >= 0 && ns >= 0, "update_pair: negative size");
+    const int n = nf + ns;
+    if (n == 0) return HHS_OK;
+    hipLaunchKernelGGL((update_pair_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s, prev,
+                       fast, nf, slow, ns, h_slow);
+    HHS_LAUNCH_CHECK("update_pair_kernel");
+    return HHS_OK;
+}
+template int launch_update_pair<double>(const Particles<double> &, const Particles<double> &,
+                                        int, const Particles<double> &, int, double,
+                                        hipStream_t);
 
 }  // namespace hhs
 
