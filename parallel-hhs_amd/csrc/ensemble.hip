@@ -796,14 +796,15 @@ __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<doubl
         total.m[u] = a.wm[g];
         total.timestep[u] = a.wts[g];
     }
+    // (out may be host-mapped memory: one store per thread)
     if (tm.t == 0) {
         out->calls = st.calls;
         out->interactions = st.interactions;
         out->ts_pairs = st.ts_pairs;
         out->max_level = st.max_level;
         out->status = status;
-        for (int l = 0; l < kMaxFrames; ++l) out->level_calls[l] = sh.level_calls[l];
     }
+    for (int l = tm.t; l < kMaxFrames; l += kWG) out->level_calls[l] = sh.level_calls[l];
 }
 
 __global__ void zero_counters_kernel(unsigned int *c) {

@@ -61,8 +61,8 @@ struct hhs_globals {
     size_t ts_ws_bytes = 0;
     void *sub_ws = nullptr;
     size_t sub_ws_bytes = 0;
-    hhs::SubtreeStats *sub_dev = nullptr;
-    hhs::SubtreeStats *sub_host = nullptr;  // pinned
+    // subtree statistics, written by the kernel into host-mapped memory
+    hhs::SubtreeStats *sub_host = nullptr, *sub_map = nullptr;
 };
 
 namespace hhs {
@@ -128,10 +128,8 @@ bool use_subtree(const hhs_globals *g, int n, int clevel) {
 int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
                 bool calc, hipStream_t st) {
     int rc = launch_subtree(total, n, dt, calc ? 1 : 0, g->eta, g->eps2, g->timestep_mode,
-                            g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_dev, st);
+                            g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_map, st);
     if (rc) return rc;
-    HHS_HIP(hipMemcpyAsync(g->sub_host, g->sub_dev, sizeof(SubtreeStats), hipMemcpyDeviceToHost,
-                           st));
     HHS_HIP(hipStreamSynchronize(st));
     const SubtreeStats &o = *g->sub_host;
     hhs_hold_stats &S = g->stats;
@@ -158,10 +156,13 @@ int alloc_subtree(hhs_globals *g, int n_max) {
     if (n_max <= 0) return HHS_OK;
     const int cap = n_max < g->n_total ? n_max : g->n_total;
     const size_t bytes = subtree_workspace_bytes(cap);
-    if (!g->sub_dev) HHS_HIP(hipMalloc(&g->sub_dev, sizeof(SubtreeStats)));
-    if (!g->sub_host)
+    if (!g->sub_host) {
         HHS_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->sub_host), sizeof(SubtreeStats),
-                              hipHostMallocDefault));
+                              hipHostMallocMapped | hipHostMallocCoherent));
+        void *map = nullptr;
+        HHS_HIP(hipHostGetDevicePointer(&map, g->sub_host, 0));
+        g->sub_map = static_cast<SubtreeStats *>(map);
+    }
     HHS_HIP(hipMalloc(&g->sub_ws, bytes));
     g->sub_ws_bytes = bytes;
     g->subtree_max = cap;
@@ -325,7 +326,6 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->part_block) (void)hipHostFree(g->part_block);
     if (g->sub_ws) (void)hipFree(g->sub_ws);
     if (g->ts_ws) (void)hipFree(g->ts_ws);
-    if (g->sub_dev) (void)hipFree(g->sub_dev);
     if (g->sub_host) (void)hipHostFree(g->sub_host);
     delete g;
     return HHS_OK;
