@@ -12,6 +12,8 @@ peak.  Beside the headline line the JSON carries:
     at N = 131072 and 2^20..2^24;
   * hold_root_step -- one HOLD root step (dt = 1/64) of the same system,
     beside the FP64-only HOLD of the same step (the energy reference);
+  * hold_n16384 -- the plain HOLD of an N = 16384 Plummer sphere to t =
+    0.125, beside its FP64-only run;
   * ensemble -- 4096 Plummer systems (N log-uniform in [8, 2048], t = 1) in
     one persistent launch, against the plain one-system-at-a-time loop;
   * served -- a Poisson request stream through the continuous-batching
@@ -180,24 +182,46 @@ def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
     return out
 
 
-def hold_bench(hhs, torch, pos, vel, m, dt, label, **modes):
-    """One HOLD root step of the N = 131072 system; returns the summary and
-    the final positions (for the comparison with the FP64-only run)."""
+def hold_bench(hhs, torch, pos, vel, m, dt, label, eps=EPS, t_end=None, **modes):
+    """One HOLD root step of dt (or root steps of dt to t_end) of the given
+    system; returns the summary and the final positions (for the comparison
+    with the FP64-only run)."""
     d2 = hhs.DeviceParticles.from_host(pos, vel, m)
-    g = hhs.Globals(N, ETA, EPS, **modes)
-    ke0, pe0 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
+    g = hhs.Globals(len(m), ETA, eps, **modes)
+    ke0, pe0 = hhs.energy(d2.pos, d2.vel, d2.m, eps * eps)
     torch.cuda.synchronize()
     h0 = time.perf_counter()
-    g.hold_step(d2, dt)
+    if t_end is None:
+        g.hold_step(d2, dt)
+    else:
+        g.hold(d2, t_end, dt)
     torch.cuda.synchronize()
     hs = time.perf_counter() - h0
-    ke1, pe1 = hhs.energy(d2.pos, d2.vel, d2.m, EPS * EPS)
+    ke1, pe1 = hhs.energy(d2.pos, d2.vel, d2.m, eps * eps)
     st = g.stats()
     g.close()
     return {"dt": dt, "mode": label, "seconds": round(hs, 4), "calls": st["calls"],
             "levels": st["max_level"] + 1, "interactions": st["interactions"],
             "ts_pairs": st["ts_pairs"],
             "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0))}, d2.pos.cpu().numpy()
+
+
+def hold16k_bench(hhs, torch):
+    """Plain HOLD of the N = 16384 equal-mass Plummer sphere (seed 2, eps =
+    1e-4, eta = 0.01) to t = 0.125 in one root step (about ten power-of-two
+    levels): the default mode (MIXED32 kicks, FP32 timesteps) beside the
+    FP64-only mode, which is bit-identical with the FP64 CPU HOLD and gives
+    the reference energy error."""
+    n, eps, t_end = 16384, 1e-4, 0.125
+    pos, vel, m = hhs.ic_plummer(n, 2, 0, True, eps * eps)
+    out, p_def = hold_bench(hhs, torch, pos, vel, m, t_end, "mixed kick, FP32 timesteps",
+                            eps=eps, fast_timesteps=hhs.TIMESTEP_FP32)
+    ref, p64 = hold_bench(hhs, torch, pos, vel, m, t_end, "FP64-only (exact kick and timesteps)",
+                          eps=eps, kick_precision=hhs.KICK_F64_EXACT)
+    ref["max_pos_diff_over_rms"] = float(np.abs(p_def - p64).max() / np.sqrt((p64 ** 2).mean()))
+    out.update({"n": n, "eps": eps, "t_end": t_end, "fp64_reference": ref,
+                "dE_ratio_to_fp64": out["dE_over_E"] / ref["dE_over_E"]})
+    return out
 
 
 def timesteps_bench(hhs, torch, d, stream, reps=3):
@@ -533,6 +557,9 @@ def main():
                                                  / np.sqrt((p64 ** 2).mean()))
             out["hold_root_step"]["fp64_reference"] = ref
             log(f"hold fp64: {ref}")
+    if not args.no_hold:
+        out["hold_n16384"] = hold16k_bench(hhs, torch)
+        log(f"hold n=16384: {out['hold_n16384']}")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample)
         log(f"ensemble: {out['ensemble']}")
