@@ -8,6 +8,8 @@ Ginteractions/s over the timed steps (inputs resident in HBM).  The roofline
 object prices the kick at 20 flops per interaction against the FP32 vector
 peak.  Beside the headline line the JSON carries:
   * timesteps -- the timestep kernel over the same system, three modes;
+  * timestep_autotune -- the one-sided FP32 timestep kernel tuned for the
+    size classes 2^8 .. 2^15 (the HOLD legs then run with it);
   * stream_kernels -- split, scan, drift and update against HBM bandwidth
     at N = 131072 and 2^20..2^24;
   * hold_root_step -- one HOLD root step (dt = 1/64) of the same system,
@@ -158,6 +160,7 @@ def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
                         "note": "timing includes one copy of the input array"}
     oracle.set_threads(1)
     p1, v1, m1 = hhs.ic_plummer(1024, 1, 0, True, 0.0)
+    e0 = oracle.energy(p1, v1, m1, 0.0)
     host = oracle.HostParticles(1024, p1, v1, m1)
     t0 = time.perf_counter()
     rc, ost = oracle.hold(host, 1.0, 1.0 / 16, ETA, 0.0)
@@ -170,7 +173,9 @@ def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
         t0 = time.perf_counter()
         g1.hold(d1, 1.0, 1.0 / 16)
         torch.cuda.synchronize()
-        gpu[label] = {"seconds": round(time.perf_counter() - t0, 3), "calls": g1.stats()["calls"]}
+        gpu[label] = {"seconds": round(time.perf_counter() - t0, 3), "calls": g1.stats()["calls"],
+                      "dE_over_E": abs(oracle.energy(d1.pos.cpu().numpy(), d1.vel.cpu().numpy(),
+                                                     m1, 0.0) / e0 - 1.0)}
         if label == "fp64":
             gpu[label]["bit_identical_to_cpu"] = bool(
                 d1.pos.cpu().numpy().tobytes() == host.pos.tobytes()
@@ -178,7 +183,10 @@ def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
         g1.close()
     out["hold_n1024"] = {"cpu_seconds": round(t_cpu, 3), "cpu_threads": 1, "rc": rc,
                          "calls": ost.calls, "levels": ost.max_level + 1,
-                         "interactions": ost.interactions, "gpu": gpu}
+                         "interactions": ost.interactions,
+                         "dE_over_E": abs(oracle.energy(host.pos, host.vel, host.m, 0.0) / e0
+                                          - 1.0),
+                         "gpu": gpu}
     return out
 
 
@@ -222,6 +230,30 @@ def hold16k_bench(hhs, torch):
     out.update({"n": n, "eps": eps, "t_end": t_end, "fp64_reference": ref,
                 "dE_ratio_to_fp64": out["dE_over_E"] / ref["dE_over_E"]})
     return out
+
+
+def tune_timesteps(hhs, torch, sizes=tuple(1 << k for k in range(8, 16)), reps=5):
+    """Autotunes the one-sided FP32 timestep kernel (waves per group, source
+    chunks) for the size classes 2^8 .. 2^15 that HOLD's levels below the
+    symmetric kernel use; reports the heuristic's time beside the tuned one."""
+    rows = []
+    for n in sizes:
+        p, v, m = hhs.ic_plummer(n, 777, 0, True, EPS * EPS)
+        d = hhs.DeviceParticles.from_host(p, v, m)
+        hw, hk = hhs.timesteps_get_config(n, hhs.TIMESTEP_FP32)
+        for _ in range(2):
+            hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=hhs.TIMESTEP_FP32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hhs.timesteps(d, eta=ETA, eps2=EPS * EPS, fast=hhs.TIMESTEP_FP32)
+        torch.cuda.synchronize()
+        h_ms = (time.perf_counter() - t0) / reps * 1e3
+        w, k, ms = hhs.autotune_timesteps(n, hhs.TIMESTEP_FP32, reps)
+        rows.append({"n": n, "heuristic": [hw, hk], "heuristic_ms": round(h_ms, 4),
+                     "tuned": [w, k], "tuned_ms": round(ms, 4)})
+        del d
+    return {"mode": "fp32", "config": "[waves per group, source chunks]", "classes": rows}
 
 
 def timesteps_bench(hhs, torch, d, stream, reps=3):
@@ -545,6 +577,9 @@ def main():
     del d
     if not args.no_stream:
         out["stream_kernels"] = stream_bench(hhs, torch, stream)
+    if not args.no_tune:
+        out["timestep_autotune"] = tune_timesteps(hhs, torch)
+        log(f"timestep autotune: {out['timestep_autotune']}")
     if not args.no_hold:
         out["hold_root_step"], p_mixed = hold_bench(
             hhs, torch, pos, vel, m, dt, "mixed kick, FP32 timesteps",

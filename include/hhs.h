@@ -155,6 +155,21 @@ int hhs_kick_get_precision(void);
  * hhs_kick_set_config overrides tuned entries. */
 int hhs_autotune_kick(int n, int reps, const char *cache_path, int *best_s, int *best_w,
                       double *best_ms);
+/* Timestep autotuner (no reference counterpart).  Times the one-sided
+ * timestep kernel of `mode` (HHS_TIMESTEP_*) at every (waves per group W,
+ * source chunks K) that keeps each wave >= 8 sources, on a synthetic
+ * n-particle FP64 system (reps timed launches each, default stream,
+ * blocking), installs the fastest for that mode and n's power-of-two size
+ * class (FP64 storage), and appends "ts <mode> <class> <W> <K> <ms>" to
+ * cache_path when it is non-NULL.  Minima do not depend on the split, so
+ * EXACT stays bit-exact under any tuned configuration. */
+int hhs_autotune_timesteps(int n, int mode, int reps, const char *cache_path, int *best_w,
+                           int *best_k, double *best_ms);
+/* The (waves per group, source chunks) the one-sided timestep kernel of
+ * `mode` uses for n particles in FP64 storage: tuned, or the heuristic. */
+int hhs_timesteps_get_config(int n, int mode, int *waves_per_group, int *chunks);
+/* Installs the "kick" and "ts" entries of a cache file (later lines win);
+ * hhs_autotune_clear returns both to the built-in heuristics. */
 int hhs_autotune_load(const char *cache_path);
 int hhs_autotune_clear(void);
 int hhs_kick_get_config(int n_sink, int n_src, int *sinks_per_lane, int *waves_per_group);

@@ -105,6 +105,10 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t s);
 template <typename T>
 int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
                      hipStream_t s);
+// Autotuned one-sided timestep configurations (timestep.hip).
+void ts_tuned_install(int mode, int cls, int w, int k);
+void ts_tuned_clear();
+bool ts_tuned_valid(int mode, int cls, int w, int k);
 int launch_timesteps_mode(int mode, const double *pos, const double *vel, const double *m,
                           double *ts, int n, double eta, double eps2, hipStream_t s);
 // Symmetric evaluation (each unordered pair once) for the EXACT and FP32

@@ -621,13 +621,23 @@ int hhs_autotune_load(const char *cache_path) {
         set_error("autotune_load: cannot open %s", cache_path);
         return HHS_ERR_INVALID;
     }
-    char kind[16];
-    int cls, s, w;
+    // "kick <class> <S> <W> <ms>" and "ts <mode> <class> <W> <K> <ms>";
+    // later lines win
+    char line[256], kind[16];
+    int a0, a1, a2, a3;
     double ms;
     int loaded = 0;
-    while (std::fscanf(f, "%15s %d %d %d %lf", kind, &cls, &s, &w, &ms) == 5) {
-        if (std::strcmp(kind, "kick") == 0 && cls >= 0 && cls < 32 && valid_s(s) && valid_w(w)) {
-            g_tuned[cls].store(s << 8 | w);  // later lines win
+    while (std::fgets(line, sizeof(line), f)) {
+        if (std::sscanf(line, "%15s", kind) != 1) continue;
+        if (std::strcmp(kind, "kick") == 0 &&
+            std::sscanf(line, "%*s %d %d %d %lf", &a0, &a1, &a2, &ms) == 4 && a0 >= 0 &&
+            a0 < 32 && valid_s(a1) && valid_w(a2)) {
+            g_tuned[a0].store(a1 << 8 | a2);
+            ++loaded;
+        } else if (std::strcmp(kind, "ts") == 0 &&
+                   std::sscanf(line, "%*s %d %d %d %d %lf", &a0, &a1, &a2, &a3, &ms) == 5 &&
+                   ts_tuned_valid(a0, a1, a2, a3)) {
+            ts_tuned_install(a0, a1, a2, a3);
             ++loaded;
         }
     }
@@ -638,6 +648,7 @@ int hhs_autotune_load(const char *cache_path) {
 
 int hhs_autotune_clear(void) {
     for (auto &t : g_tuned) t.store(0);
+    ts_tuned_clear();
     return HHS_OK;
 }
 

@@ -14,7 +14,10 @@
 // The reference's early `return` before a barrier (src/timestep.cuh:70 vs
 // :89,:95) has no counterpart: there are no barriers here, and lanes past
 // n compute on a clamped sink and do not store.
+#include <atomic>
+#include <cstdio>
 #include <type_traits>
+#include <vector>
 
 #include "hhs_internal.h"
 #include "pair_math.h"
@@ -130,16 +133,17 @@ int pick_waves(int n) {
 }
 
 // Split-K as for the kick: more workgroups over chunks of the sources
-// while the grid stays under 8192 waves, each wave keeps >= kTsMinChunk
-// sources and the partials (two values per sink and chunk) stay within
-// 4 MB.  MI355X, FP32 mode: n = 4096 143 -> 33 us, 8192 283 -> 94 us, 16384
-// 577 -> 340 us.
+// while the grid stays within kTsMaxWaves waves, each wave keeps >=
+// kTsMinChunk sources and the partials (two values per sink and chunk) stay
+// within 4 MB.  The limits follow the autotuner's choices on MI355X (FP32:
+// n = 8192 K 8 -> 16, 16384 4 -> 8, 32768 2 -> 4, 15-20 % faster).
 constexpr int kTsMinN = 256, kTsMinChunk = 16;
+constexpr long long kTsMaxWaves = 16384;
 template <typename V> int pick_ts_split(int n, int W) {
     const long long tiles = (n + 63) / 64;
     int K = 1;
     if (n < kTsMinN) return K;
-    while (K < 64 && tiles * W * K * 2 <= 8192 && n / (W * K * 2) >= kTsMinChunk &&
+    while (K < 64 && tiles * W * K * 2 <= kTsMaxWaves && n / (W * K * 2) >= kTsMinChunk &&
            2ULL * sizeof(V) * n * K * 2 <= (size_t{4} << 20))
         K *= 2;
     return K;
@@ -152,6 +156,35 @@ void launch_ts_w(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T 
                        s, pos, vel, m, ts, n, eta, eps2, K, K > 1 ? part : nullptr);
 }
 
+// Autotuned (W, K) per mode and sink-count size class (floor(log2 n)),
+// encoded W << 8 | K (0 = the heuristic above), and the sweep's override.
+std::atomic<int> g_ts_tuned[3][32];
+std::atomic<int> g_ts_force_w{0}, g_ts_force_k{0};
+int ts_size_class(int n) { return n <= 1 ? 0 : 31 - __builtin_clz(static_cast<unsigned>(n)); }
+bool valid_ts_w(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
+bool valid_ts_k(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
+// A tuned K is used while its partials stay within 64 MB.
+constexpr size_t kTsMaxTunedPartial = size_t{64} << 20;
+
+// The launch configuration: the heuristic, or the tuned (or forced) entry
+// for FP64 storage while its partials stay within the cap.
+template <typename T, int MODE> void ts_config(int n, int &W, int &K) {
+    using V = typename TsAcc<T, MODE>::V;
+    W = pick_waves(n);
+    K = pick_ts_split<V>(n, W);
+    if constexpr (std::is_same<T, double>::value) {
+        const int t = g_ts_tuned[MODE][ts_size_class(n)].load();
+        int w = t >> 8, k = t & 0xff;
+        if (g_ts_force_w.load()) w = g_ts_force_w.load();
+        if (g_ts_force_k.load()) k = g_ts_force_k.load();
+        if (valid_ts_w(w) && valid_ts_k(k) &&
+            2ULL * sizeof(V) * static_cast<size_t>(n) * k <= kTsMaxTunedPartial) {
+            W = w;
+            K = k;
+        }
+    }
+}
+
 template <typename T, int MODE>
 int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
               hipStream_t s) {
@@ -159,8 +192,8 @@ int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps
     HHS_REQUIRE(n >= 0, "timesteps: negative n");
     if (n == 0) return HHS_OK;
     HHS_REQUIRE(pos && vel && m && ts, "timesteps: null array");
-    const int W = pick_waves(n);
-    const int K = pick_ts_split<V>(n, W);
+    int W, K;
+    ts_config<T, MODE>(n, W, K);
     static thread_local ThreadScratch scratch;
     V *part = nullptr;
     if (K > 1) {
@@ -186,6 +219,16 @@ int launch_ts(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps
 }
 
 }  // namespace
+
+void ts_tuned_install(int mode, int cls, int w, int k) { g_ts_tuned[mode][cls].store(w << 8 | k); }
+void ts_tuned_clear() {
+    for (auto &row : g_ts_tuned)
+        for (auto &t : row) t.store(0);
+}
+bool ts_tuned_valid(int mode, int cls, int w, int k) {
+    return mode >= HHS_TIMESTEP_EXACT && mode <= HHS_TIMESTEP_FP32 && cls >= 0 && cls < 32 &&
+           valid_ts_w(w) && valid_ts_k(k);
+}
 
 template <typename T>
 int launch_timesteps(const T *pos, const T *vel, const T *m, T *ts, int n, T eta, T eps2,
@@ -432,6 +475,93 @@ int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2
     HHS_REQUIRE(n >= 0 && n <= sys->n, "timesteps: n=%d outside [0, %d]", n, sys->n);
     return launch_timesteps<float>(sys->pos, sys->vel, sys->m, sys->timestep, n, eta, eps2,
                                    as_stream(stream));
+}
+
+int hhs_autotune_timesteps(int n, int mode, int reps, const char *cache_path, int *best_w,
+                           int *best_k, double *best_ms) {
+    HHS_REQUIRE(n >= 64, "autotune_timesteps: n must be >= 64");
+    HHS_REQUIRE(mode >= HHS_TIMESTEP_EXACT && mode <= HHS_TIMESTEP_FP32,
+                "autotune_timesteps: mode must be HHS_TIMESTEP_EXACT, _FAST or _FP32");
+    if (reps <= 0) reps = 3;
+    const size_t nn = static_cast<size_t>(n);
+    std::vector<double> p(3 * nn), v(3 * nn), mm(nn);
+    int rc = hhs_ic_plummer(n, 54321, 0, 1, 1e-8, p.data(), v.data(), mm.data());
+    if (rc) return rc;
+    double *dev = nullptr;
+    HHS_HIP(hipMalloc(&dev, nn * 64));  // pos, vel (3n each), m, timestep
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemcpy(dev, p.data(), nn * 24, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dev + 3 * nn, v.data(), nn * 24, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dev + 6 * nn, mm.data(), nn * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const long long tiles = (n + 63) / 64;
+    const size_t vbytes = mode == HHS_TIMESTEP_FP32 ? sizeof(float) : sizeof(double);
+    double best = 1e300;
+    int bw = 0, bk = 0;
+    for (int w : {1, 2, 4, 8}) {
+        for (int k : {1, 2, 4, 8, 16, 32, 64}) {
+            if (e != hipSuccess || rc) break;
+            // keep every wave >= 8 sources, the grid <= 16384 waves and the
+            // partials within the tuned cap
+            if (n / (w * k) < 8 || tiles * w * k > 16384 ||
+                2 * vbytes * nn * k > kTsMaxTunedPartial)
+                continue;
+            g_ts_force_w.store(w);
+            g_ts_force_k.store(k);
+            auto run = [&]() {
+                return launch_timesteps_mode(mode, dev, dev + 3 * nn, dev + 6 * nn, dev + 7 * nn,
+                                             n, 0.01, 1e-8, nullptr);
+            };
+            rc = run();  // warm-up
+            if (!rc) e = hipEventRecord(e0, nullptr);
+            for (int r = 0; r < reps && !rc && e == hipSuccess; ++r) rc = run();
+            if (!rc && e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (!rc && e == hipSuccess && ms / reps < best) {
+                best = ms / reps;
+                bw = w;
+                bk = k;
+            }
+        }
+    }
+    g_ts_force_w.store(0);
+    g_ts_force_k.store(0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_status(e, "autotune_timesteps", __FILE__, __LINE__);
+    if (rc) return rc;
+    const int cls = ts_size_class(n);
+    ts_tuned_install(mode, cls, bw, bk);
+    if (cache_path) {
+        FILE *f = std::fopen(cache_path, "a");
+        if (!f) {
+            set_error("autotune_timesteps: cannot append to %s", cache_path);
+            return HHS_ERR_INVALID;
+        }
+        std::fprintf(f, "ts %d %d %d %d %.6f\n", mode, cls, bw, bk, best);
+        std::fclose(f);
+    }
+    if (best_w) *best_w = bw;
+    if (best_k) *best_k = bk;
+    if (best_ms) *best_ms = best;
+    return HHS_OK;
+}
+
+int hhs_timesteps_get_config(int n, int mode, int *waves_per_group, int *chunks) {
+    HHS_REQUIRE(waves_per_group && chunks, "timesteps_get_config: null output");
+    HHS_REQUIRE(n > 0, "timesteps_get_config: n must be positive");
+    switch (mode) {
+    case HHS_TIMESTEP_EXACT: ts_config<double, HHS_TIMESTEP_EXACT>(n, *waves_per_group, *chunks); break;
+    case HHS_TIMESTEP_FAST: ts_config<double, HHS_TIMESTEP_FAST>(n, *waves_per_group, *chunks); break;
+    case HHS_TIMESTEP_FP32: ts_config<double, HHS_TIMESTEP_FP32>(n, *waves_per_group, *chunks); break;
+    default: set_error("timesteps_get_config: unknown mode %d", mode); return HHS_ERR_INVALID;
+    }
+    return HHS_OK;
 }
 
 }  // extern "C"

@@ -145,6 +145,9 @@ _SIGS = {
     "hhs_kick_get_precision": (_i, []),
     "hhs_autotune_kick": (_i, [_i, _i, C.c_char_p, C.POINTER(_i), C.POINTER(_i),
                                C.POINTER(_d)]),
+    "hhs_autotune_timesteps": (_i, [_i, _i, _i, C.c_char_p, C.POINTER(_i), C.POINTER(_i),
+                                    C.POINTER(C.c_double)]),
+    "hhs_timesteps_get_config": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
     "hhs_autotune_load": (_i, [C.c_char_p]),
     "hhs_autotune_clear": (_i, []),
     "hhs_kick_get_config": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
@@ -325,6 +328,23 @@ def autotune_kick(n: int, reps: int = 3, cache_path: str | None = None):
     path = cache_path.encode() if cache_path else None
     check(_lib.hhs_autotune_kick(n, reps, path, C.byref(s), C.byref(w), C.byref(ms)))
     return s.value, w.value, ms.value
+
+
+def autotune_timesteps(n: int, mode: int, reps: int = 3, cache_path: str | None = None):
+    """Sweeps (waves per group, source chunks) of the one-sided timestep
+    kernel for `mode` at n particles; returns (W, K, ms) and installs it."""
+    w, k, ms = C.c_int(), C.c_int(), C.c_double()
+    path = None if cache_path is None else cache_path.encode()
+    check(_lib.hhs_autotune_timesteps(n, _ts_mode(mode), reps, path, C.byref(w), C.byref(k),
+                                      C.byref(ms)))
+    return w.value, k.value, ms.value
+
+
+def timesteps_get_config(n: int, mode: int):
+    """(waves per group, source chunks) of the one-sided timestep kernel."""
+    w, k = C.c_int(), C.c_int()
+    check(_lib.hhs_timesteps_get_config(n, _ts_mode(mode), C.byref(w), C.byref(k)))
+    return w.value, k.value
 
 
 def autotune_load(cache_path: str):

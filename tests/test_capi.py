@@ -143,3 +143,25 @@ def test_autotune_cache_round_trip(hhs, tmp_path):
     bad.write_text("nothing here\n")
     with pytest.raises(hhs.HHSError):
         hhs.autotune_load(str(bad))
+
+
+def test_timestep_autotune_cache_round_trip(hhs, tmp_path):
+    """"ts <mode> <class> <W> <K> <ms>" lines install one-sided timestep
+    configurations per mode and size class; invalid entries are ignored and
+    clearing returns to the heuristic (host logic only, no GPU)."""
+    base_fp32 = hhs.timesteps_get_config(3000, hhs.TIMESTEP_FP32)
+    base_exact = hhs.timesteps_get_config(3000, hhs.TIMESTEP_EXACT)
+    path = tmp_path / "tune.txt"
+    path.write_text("ts 2 11 2 16 0.01\nts 2 11 4 8 0.009\nts 0 11 8 2 0.03\n"
+                    "ts 2 11 3 8 0.001\nts 7 11 2 2 0.001\nkick 12 1 8 0.5\n")
+    try:
+        hhs.autotune_load(str(path))
+        assert hhs.timesteps_get_config(3000, hhs.TIMESTEP_FP32) == (4, 8)   # later line wins
+        assert hhs.timesteps_get_config(2048, hhs.TIMESTEP_EXACT) == (8, 2)  # same class 11
+        assert hhs.timesteps_get_config(4096, hhs.TIMESTEP_FP32) == \
+            hhs.timesteps_get_config(4096, hhs.TIMESTEP_FAST)                # class 12 untouched
+        assert hhs.kick_get_config(4096, 4096) == (1, 8)
+    finally:
+        hhs.autotune_clear()
+    assert hhs.timesteps_get_config(3000, hhs.TIMESTEP_FP32) == base_fp32
+    assert hhs.timesteps_get_config(3000, hhs.TIMESTEP_EXACT) == base_exact

@@ -180,3 +180,29 @@ def test_timesteps_symmetric_large(hhs, oracle, torch_cuda, n):
     ta, tb, tc = (x.timestep.cpu().numpy() for x in (a, b, c))
     assert ta.tobytes() == tb.tobytes()
     assert np.max(np.abs(tc / ta - 1.0)) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_timestep_autotune_installs_a_valid_config(hhs, oracle, torch_cuda, tmp_path, mode):
+    """The timestep autotuner installs a (W, K) for the size class and mode,
+    writes it to the cache, and results keep their bar under it: EXACT bit
+    for bit, FP32 within 1e-6."""
+    path = tmp_path / "tune.txt"
+    try:
+        w, k, ms = hhs.autotune_timesteps(2048, mode, 2, str(path))
+        assert w in (1, 2, 4, 8) and k in (1, 2, 4, 8, 16, 32, 64) and ms > 0
+        assert hhs.timesteps_get_config(3000, mode) == (w, k)   # same size class (11)
+        fields = path.read_text().split()
+        assert fields[:5] == ["ts", str(mode), "11", str(w), str(k)]
+        pos, vel, m = plummer(hhs, 3000, seed=77, eps2=1e-8)
+        d = hhs.DeviceParticles.from_host(pos, vel, m)
+        hhs.timesteps(d, eta=0.01, eps2=1e-8, fast=mode)
+        torch_cuda.cuda.synchronize()
+        ref = oracle.timesteps(pos, vel, m, 0.01, 1e-8)
+        got = d.timestep.cpu().numpy()
+        if mode == 0:
+            assert got.tobytes() == ref.tobytes()
+        else:
+            assert np.all(np.abs(got - ref) <= 1e-6 * ref)
+    finally:
+        hhs.autotune_clear()
