@@ -366,7 +366,7 @@ def stream_bench(hhs, torch, stream, sizes=(N, 1 << 20, 1 << 22, 1 << 24), reps=
             "peak_GB_per_s": HBM_PEAK_GBS, "sizes": rows}
 
 
-def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
+def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12, fp64_sample=6):
     """Ensemble workload: 4096 Plummer systems, N log-uniform in [8, 2048],
     equal masses, eta = 0.01, t = 1, eps = 1e-4, root steps of 1/16, FP32
     timesteps, one persistent launch."""
@@ -400,6 +400,23 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
         g.close()
     loop_per_system = float(np.mean(loop_t)) if loop_t else float("nan")
     sps = n_systems / wall
+    # Systems that miss |dE/E| <= 1e-6, integrated again one at a time by the
+    # FP64-only HOLD (exact kick and timesteps): the same error there means it
+    # is the integrator's at this eta, not the mixed precision's.
+    fail = np.nonzero(ok & (de > 1e-6) & (sizes <= 400))[0]
+    fail = fail[np.argsort(sizes[fail])]
+    fp64_check = []
+    for k in (fail[np.linspace(0, len(fail) - 1, fp64_sample).astype(int)]
+              if len(fail) and fp64_sample > 0 else []):
+        p, v, mm = systems[k]
+        dd = hhs.DeviceParticles.from_host(p, v, mm)
+        g = hhs.Globals(len(mm), ETA, EPS, kick_precision=hhs.KICK_F64_EXACT)
+        e0 = sum(hhs.energy(dd.pos, dd.vel, dd.m, EPS * EPS))
+        g.hold(dd, t_end, dtr)
+        e1 = sum(hhs.energy(dd.pos, dd.vel, dd.m, EPS * EPS))
+        g.close()
+        fp64_check.append({"n": int(sizes[k]), "ensemble_dE": float(de[k]),
+                           "fp64_dE": abs((e1 - e0) / e0)})
     return {"systems": n_systems, "n_range": [8, 2048], "t_end": t_end, "dt_root": dtr,
             "eta": ETA, "eps": EPS, "timesteps": "fp32", "seconds": round(wall, 3),
             "systems_per_s": round(sps, 2),
@@ -413,7 +430,11 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12):
                            "sample": "evenly spaced in sorted N (stratified)",
                            "seconds_per_system": round(loop_per_system, 4),
                            "systems_per_s": round(1.0 / loop_per_system, 3) if loop_t else None},
-            "speedup_vs_plain_loop": round(sps * loop_per_system, 1) if loop_t else None}
+            "speedup_vs_plain_loop": round(sps * loop_per_system, 1) if loop_t else None,
+            "fp64_check_of_misses": {
+                "sample": "systems with N <= 400 that miss 1e-6, evenly spaced in N, "
+                          "rerun by the FP64-only HOLD (exact kick and timesteps)",
+                "systems": fp64_check}}
 
 
 def serve_bench(hhs, rates, n_req, n_lo, n_hi, seed=5, batch_streams=0, big_workers=0,
