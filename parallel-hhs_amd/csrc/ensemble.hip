@@ -12,9 +12,12 @@
 //   first): a "big" system takes the whole workgroup, a bundle of four
 //   "small" systems one wave each.
 // * A system's numbers depend only on its own data, not on the team size
-//   (per-sink source order, 64-source kick blocks from the range start, a
-//   stable partition, energies from the first wave), so a system integrates
-//   to the same bits alone or in any mix, on a wave or a workgroup team.
+//   (per-sink source order; a call with <= 256 sources sums and minimises
+//   over four source quarters combined in order, a larger one over
+//   64-source kick blocks from the range start; a stable partition;
+//   energies from the first wave), so a system integrates to the same bits
+//   alone or in any mix, on a wave or a workgroup team, and a device
+//   subtree to the same bits as the ensemble.
 // * In-place hierarchy: a recursion level owns a contiguous index range
 //   [b, e) of the system's working copy; the split partitions it stably,
 //   fast first, through a scratch copy.  No scatter-back is needed; the
@@ -34,6 +37,9 @@ namespace {
 
 constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
+// Team of an LDS-resident device subtree (<= kSubLds particles): four
+// waves per SIMD, so the one workgroup on the GPU hides its own latency.
+constexpr int kSubTeam = 1024;
 constexpr int kMaxFrames = kSubtreeLevels;  // max_level <= 47 plus root and one child
 constexpr int kFold = 64;
 constexpr int kCounters = 16;  // queue counters in the workspace
@@ -274,29 +280,270 @@ template <int TEAM> __device__ __forceinline__ int sinks_per_lane(int rem, int c
     return min(cap, (rem + TEAM - 1) / TEAM);
 }
 
+// ------------------------------------------------ calls with few sources
+// A call whose source range holds at most kSmallSrc particles (the deep
+// levels of every system, and every call of a device subtree) has few
+// sinks too, so one lane per sink would leave most of a workgroup idle:
+// at 50 sinks one wave does all the pair work while three wait.  Such calls
+// use one fixed sum structure instead: the sources [jb, je) are cut into
+// kQuarters quarters of ceil(L / kQuarters); each quarter's minimum (a
+// fresh fold from 1e30) or force (a plain FP32 sum from zero) is formed on
+// its own, and the quarters are combined in quarter order (a fold of the
+// fractions; FP64 adds of the sums).  A workgroup team with <= 128 sinks
+// forms the quarters in parallel, one or two per wave, and combines them
+// through LDS; other teams form them one after the other.  The bits are
+// the same either way.
+constexpr int kQuarters = 4;
+constexpr int kSmallSrc = kWG;  // the whole source range fits one workgroup tile
+// A workgroup team's quarter partials sit in its LDS tile array after the
+// first kTileDoubles.  Groups of 64 sinks whose quarters are spread over
+// waves (2 waves or more per group) number at most kMaxSplitGroups; the
+// area holds two kicks' FP32 quarter sums or one pass's quarter minima.
+constexpr int kTileDoubles = kWG * kTsStride;
+template <int TEAM> constexpr int kMaxSplitGroups = TEAM / 128 < 4 ? TEAM / 128 : 4;
+template <int TEAM> constexpr int kPartDoubles = 2 * kMaxSplitGroups<TEAM> * kQuarters * 3 * 64 / 2;
+
+struct Span {
+    int b, e;
+};
+__device__ __forceinline__ Span quarter(int jb, int je, int q) {
+    const int qs = (je - jb + kQuarters - 1) / kQuarters;
+    const int b = min(jb + q * qs, je);
+    return Span{b, min(b + qs, je)};
+}
+
+// Running minimum of one sink in the representation of each mode (FP32 and
+// FAST: a fraction num/den; EXACT: the value), as in team_timesteps_pass.
+template <int MODE> struct TsFold {
+    using V = typename std::conditional<MODE == HHS_TIMESTEP_FP32, float, double>::type;
+    V num, den;
+    __device__ void init() {
+        num = static_cast<V>(TsConst<double>::huge);
+        den = V(1);
+    }
+    __device__ void merge(V onum, V oden) {
+        if constexpr (MODE == HHS_TIMESTEP_FP32)
+            fold_fraction32(onum, oden, true, num, den);
+        else if constexpr (MODE == HHS_TIMESTEP_FAST)
+            fold_fraction(onum, oden, true, num, den);
+        else if (onum < num)
+            num = onum;
+    }
+    __device__ double value() const {
+        if constexpr (MODE == HHS_TIMESTEP_FP32)
+            return static_cast<double>(num) / static_cast<double>(den);
+        else if constexpr (MODE == HHS_TIMESTEP_FAST)
+            return num / den;
+        else
+            return num;
+    }
+};
+
+struct TsSink {
+    double x, y, z, vx, vy, vz, m;
+    float mf;
+    int i;
+};
+
+__device__ __forceinline__ TsSink ts_sink(const EnsArgs &a, int i, int je) {
+    const int ic = i < je ? i : je - 1;
+    TsSink s;
+    s.x = a.wpos[3 * ic];
+    s.y = a.wpos[3 * ic + 1];
+    s.z = a.wpos[3 * ic + 2];
+    s.vx = a.wvel[3 * ic];
+    s.vy = a.wvel[3 * ic + 1];
+    s.vz = a.wvel[3 * ic + 2];
+    s.m = a.wm[ic];
+    s.mf = static_cast<float>(s.m);
+    s.i = i;
+    return s;
+}
+
+// acc[s] <- the fold, from 1e30, of sources [qb, qe) (tile entry j - t0) for
+// S sinks.
+template <int MODE, int S>
+__device__ __forceinline__ void ts_fold_span(const EnsArgs &a, const TsSink (&s)[S],
+                                             const double *tile, int t0, int qb, int qe,
+                                             float etaf, float cff, TsFold<MODE> (&acc)[S]) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) acc[k].init();
+#pragma unroll 2
+    for (int j = qb; j < qe; ++j) {
+        const double *q = tile + kTsStride * (j - t0);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const TsSink &p = s[k];
+            const bool other = j != p.i;
+            if constexpr (MODE == HHS_TIMESTEP_FP32)
+                pair_timestep_f32(p.x, p.y, p.z, p.vx, p.vy, p.vz, p.mf, q[0], q[1], q[2], q[4],
+                                  q[5], q[6], static_cast<float>(q[3]), etaf, cff, a.eps2f, other,
+                                  acc[k].num, acc[k].den);
+            else if constexpr (MODE == HHS_TIMESTEP_FAST)
+                pair_timestep_fast(p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m, q[0], q[1], q[2], q[4],
+                                   q[5], q[6], q[3], a.eta, a.c_ff, a.eps2, other, acc[k].num,
+                                   acc[k].den);
+            else
+                pair_timestep<double>(p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m, q[0], q[1], q[2],
+                                      q[4], q[5], q[6], q[3], a.eta, a.c_ff, a.eps2, other,
+                                      acc[k].num);
+        }
+    }
+}
+
+// Wave team: S sinks per lane (i0 + lane + 64 s), the quarters one after
+// the other, from one tile when L <= 64, else each through its own tile (a
+// quarter holds <= 64 sources).
+template <int MODE, int S>
+__device__ __forceinline__ void ts_small_wave_pass(const EnsArgs &a, int jb, int je, int i0, double *tile,
+                                   const Team<64> &tm, float etaf, float cff) {
+    TsSink s[S];
+    TsFold<MODE> tot[S], acc[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        s[k] = ts_sink(a, i0 + tm.t + 64 * k, je);
+        tot[k].init();
+    }
+    const bool whole = je - jb <= 64;  // one tile holds every source
+    if (whole) load_ts_tile<64>(a, jb, je, tile, tm);
+#pragma unroll 1
+    for (int q = 0; q < kQuarters; ++q) {
+        const Span Q = quarter(jb, je, q);
+        if (!whole) load_ts_tile<64>(a, Q.b, Q.e, tile, tm);
+        ts_fold_span<MODE, S>(a, s, tile, whole ? jb : Q.b, Q.b, Q.e, etaf, cff, acc);
+#pragma unroll
+        for (int k = 0; k < S; ++k) tot[k].merge(acc[k].num, acc[k].den);
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (s[k].i < je) a.wts[s[k].i] = tot[k].value();
+}
+
+// Waves per sink group for a workgroup team with G groups of 64 sinks: 4,
+// 2 or 1 (each wave then forms 1, 2 or 4 quarters), as many as the team
+// holds; 1 beyond kMaxSplitGroups groups (the partial area's size).
+template <int TEAM> __device__ __forceinline__ int waves_per_group(int G) {
+    constexpr int nw = TEAM / 64;
+    if (G > kMaxSplitGroups<TEAM>) return 1;
+    return 4 * G <= nw ? 4 : (2 * G <= nw ? 2 : 1);
+}
+
+// Workgroup team (TEAM = kWG, or kSubTeam for LDS subtrees): the L <=
+// kSmallSrc sources in one tile (entry j - jb), one sink per lane in G =
+// ceil(L / 64) groups of 64.  With R = waves_per_group > 1, the R waves of
+// a group form 4 / R quarters each, and the group's first wave combines
+// them from the partial area.
 template <int TEAM, int MODE>
-__device__ void team_timesteps_mode(const EnsArgs &a, int jb, int je, double *tile,
+__device__ __forceinline__ void ts_small_wg(const EnsArgs &a, int jb, int je, double *tile,
+                                            const Team<TEAM> &tm, float etaf, float cff) {
+    using V = typename TsFold<MODE>::V;
+    constexpr int GM = kMaxSplitGroups<TEAM>;
+    static_assert(2 * GM * kQuarters * 64 * sizeof(V) <= kPartDoubles<TEAM> * sizeof(double),
+                  "the quarter minima fit the partial area");
+    load_ts_tile<TEAM>(a, jb, je, tile, tm);
+    const int G = (je - jb + 63) >> 6;
+    const int w = uni(tm.t >> 6), lane = tm.t & 63;
+    const int R = waves_per_group<TEAM>(G);
+    if (R == 1) {
+        if (w < G) {
+            TsSink s[1] = {ts_sink(a, jb + 64 * w + lane, je)};
+            TsFold<MODE> tot, acc[1];
+            tot.init();
+#pragma unroll 1
+            for (int q = 0; q < kQuarters; ++q) {
+                const Span Q = quarter(jb, je, q);
+                ts_fold_span<MODE, 1>(a, s, tile, jb, Q.b, Q.e, etaf, cff, acc);
+                tot.merge(acc[0].num, acc[0].den);
+            }
+            if (s[0].i < je) a.wts[s[0].i] = tot.value();
+        }
+        return;
+    }
+    const int per = kQuarters / R, g = w / R, r = w % R;  // per: 1 or 2
+    TsSink s[1] = {ts_sink(a, jb + 64 * g + lane, je)};
+    V *pn = reinterpret_cast<V *>(tile + kTileDoubles);
+    V *pd = pn + GM * kQuarters * 64;
+    if (g < G) {
+        TsFold<MODE> acc[1];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k < per) {
+                const Span Q = quarter(jb, je, r * per + k);
+                ts_fold_span<MODE, 1>(a, s, tile, jb, Q.b, Q.e, etaf, cff, acc);
+                const int slot = (g * kQuarters + r * per + k) * 64 + lane;
+                pn[slot] = acc[0].num;
+                pd[slot] = acc[0].den;
+            }
+        }
+    }
+    tm.sync();
+    if (r == 0 && g < G) {
+        TsFold<MODE> tot;
+        tot.init();
+#pragma unroll
+        for (int q = 0; q < kQuarters; ++q) {
+            const int slot = (g * kQuarters + q) * 64 + lane;
+            tot.merge(pn[slot], pd[slot]);
+        }
+        if (s[0].i < je) a.wts[s[0].i] = tot.value();
+    }
+}
+
+template <int TEAM, int MODE>
+__device__ __forceinline__ void team_timesteps_small(const EnsArgs &a, int jb, int je, double *tile,
+                                     const Team<TEAM> &tm) {
+    const float etaf = static_cast<float>(a.eta), cff = static_cast<float>(a.c_ff);
+    if constexpr (TEAM == 64) {
+        constexpr int cap = MODE == HHS_TIMESTEP_FP32 ? kMaxSpl : 1;
+        for (int i0 = jb; i0 < je;) {
+            const int S = sinks_per_lane<64>(je - i0, cap);
+            if (S >= 4)
+                ts_small_wave_pass<MODE, (cap >= 4 ? 4 : 1)>(a, jb, je, i0, tile, tm, etaf, cff);
+            else if (S == 3)
+                ts_small_wave_pass<MODE, (cap >= 3 ? 3 : 1)>(a, jb, je, i0, tile, tm, etaf, cff);
+            else if (S == 2)
+                ts_small_wave_pass<MODE, (cap >= 2 ? 2 : 1)>(a, jb, je, i0, tile, tm, etaf, cff);
+            else
+                ts_small_wave_pass<MODE, 1>(a, jb, je, i0, tile, tm, etaf, cff);
+            i0 += S * 64;
+        }
+    } else {
+        ts_small_wg<TEAM, MODE>(a, jb, je, tile, tm, etaf, cff);
+    }
+}
+
+template <int TEAM, int MODE>
+__device__ __forceinline__ void team_timesteps_mode(const EnsArgs &a, int jb, int je, double *tile,
                                     const Team<TEAM> &tm) {
     constexpr int cap = MODE == HHS_TIMESTEP_FP32 ? kMaxSpl : 1;
-    for (int i0 = jb; i0 < je;) {
-        const int S = sinks_per_lane<TEAM>(je - i0, cap);
-        if (S >= 4) {
-            team_timesteps_pass<TEAM, (cap >= 4 ? 4 : 1), MODE>(a, jb, je, i0, tile, tm);
-        } else if (S == 3) {
-            team_timesteps_pass<TEAM, (cap >= 3 ? 3 : 1), MODE>(a, jb, je, i0, tile, tm);
-        } else if (S == 2) {
-            team_timesteps_pass<TEAM, (cap >= 2 ? 2 : 1), MODE>(a, jb, je, i0, tile, tm);
-        } else {
-            team_timesteps_pass<TEAM, 1, MODE>(a, jb, je, i0, tile, tm);
+    if constexpr (TEAM > kWG) {
+        // teams wider than kWG run LDS subtrees only: <= kSmallSrc particles
+        if (je > jb) team_timesteps_small<TEAM, MODE>(a, jb, je, tile, tm);
+    } else {
+        if (je - jb <= kSmallSrc) {
+            if (je > jb) team_timesteps_small<TEAM, MODE>(a, jb, je, tile, tm);
+            return;
         }
-        i0 += S * TEAM;
+        for (int i0 = jb; i0 < je;) {
+            const int S = sinks_per_lane<TEAM>(je - i0, cap);
+            if (S >= 4) {
+                team_timesteps_pass<TEAM, (cap >= 4 ? 4 : 1), MODE>(a, jb, je, i0, tile, tm);
+            } else if (S == 3) {
+                team_timesteps_pass<TEAM, (cap >= 3 ? 3 : 1), MODE>(a, jb, je, i0, tile, tm);
+            } else if (S == 2) {
+                team_timesteps_pass<TEAM, (cap >= 2 ? 2 : 1), MODE>(a, jb, je, i0, tile, tm);
+            } else {
+                team_timesteps_pass<TEAM, 1, MODE>(a, jb, je, i0, tile, tm);
+            }
+            i0 += S * TEAM;
+        }
     }
 }
 
 // timesteps of particles [o+b, o+e) against each other (src/timestep.cuh),
 // sources in index order through LDS tiles.
 template <int TEAM>
-__device__ void team_timesteps(const EnsArgs &a, int o, int b, int e, double *tile,
+__device__ __forceinline__ void team_timesteps(const EnsArgs &a, int o, int b, int e, double *tile,
                                const Team<TEAM> &tm) {
     const int jb = uni(o + b), je = uni(o + e);
     if (a.ts_mode == HHS_TIMESTEP_FP32)
@@ -410,17 +657,249 @@ __device__ void team_kick_pass(const EnsArgs &a, int i0, int se, int jb, int je,
     }
 }
 
-// vel[i] += h * sum_j m_j (x_j - x_i) / (r^2 + eps2)^{3/2}, sinks [sb, se),
-// sources [jb, je) (absolute indices) in index order through LDS tiles,
-// summed in FP32 over blocks of kFold sources from jb (TEAM is a multiple of
-// kFold, so tile edges fall on block edges) folded into a two-sum
-// accumulator.
+// Kick tile: sources [s0, min(s0 + TEAM, je)) as x, y, z, m (entry j - s0).
+template <int TEAM>
+__device__ __forceinline__ void load_kick_tile(const EnsArgs &a, int s0, int je, double *tile,
+                                               const Team<TEAM> &tm) {
+    tm.sync();  // the previous tile has been read by every lane
+    const int j = s0 + tm.t;
+    if (j < je) {
+        double *q = tile + kKickStride * tm.t;
+        q[0] = a.wpos[3 * j];
+        q[1] = a.wpos[3 * j + 1];
+        q[2] = a.wpos[3 * j + 2];
+        q[3] = a.wm[j];
+    }
+    tm.sync();
+}
+
+// vel[i] += a * h, the same expression on every path.
+__device__ __forceinline__ void kick_apply(const EnsArgs &a, int i, double ax, double ay,
+                                           double az, double h) {
+    a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
+    a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
+    a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
+}
+
+// Plain FP32 force sums from zero over sources [qb, qe) (tile entry j - t0)
+// for S sinks.
+template <int S, bool SOFT>
+__device__ __forceinline__ void kick_span(const EnsArgs &a, const double (&xi)[S],
+                                          const double (&yi)[S], const double (&zi)[S],
+                                          const double *tile, int t0, int qb, int qe,
+                                          float (&px)[S], float (&py)[S], float (&pz)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) px[s] = py[s] = pz[s] = 0.f;
+#pragma unroll 4
+    for (int j = qb; j < qe; ++j) {
+        const double *q = tile + kKickStride * (j - t0);
+        const float mj = static_cast<float>(q[3]);
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            pair_accel<SOFT>(q[0], q[1], q[2], mj, xi[s], yi[s], zi[s], a.eps2f, px[s], py[s],
+                             pz[s]);
+    }
+}
+
+// S sinks per lane (i0 + t + TEAM s), the quarters one after the other.
+// WHOLE: the tile already holds every source (entry j - tb); otherwise the
+// sources are loaded here, all at once when they fit one tile, else each
+// quarter into its own tile (wave teams: a quarter holds <= 64 sources).
+template <int TEAM, int S, bool SOFT, bool WHOLE>
+__device__ __forceinline__ void kick_small_pass(const EnsArgs &a, int i0, int se, int jb, int je,
+                                                int tb, double h, double *tile,
+                                                const Team<TEAM> &tm) {
+    int ii[S];
+    double xi[S], yi[S], zi[S], ax[S], ay[S], az[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        ii[s] = i0 + tm.t + s * TEAM;
+        const int ic = ii[s] < se ? ii[s] : se - 1;
+        xi[s] = a.wpos[3 * ic];
+        yi[s] = a.wpos[3 * ic + 1];
+        zi[s] = a.wpos[3 * ic + 2];
+        ax[s] = ay[s] = az[s] = 0.0;
+    }
+    // a wave team's tile holds TEAM sources: one load when they all fit
+    const bool whole = WHOLE || je - jb <= TEAM;
+    if (!WHOLE && whole) load_kick_tile<TEAM>(a, jb, je, tile, tm);
+    const int t0 = WHOLE ? tb : jb;
+#pragma unroll 1
+    for (int q = 0; q < kQuarters; ++q) {
+        const Span Q = quarter(jb, je, q);
+        if (!whole) load_kick_tile<TEAM>(a, Q.b, Q.e, tile, tm);
+        float px[S], py[S], pz[S];
+        kick_span<S, SOFT>(a, xi, yi, zi, tile, whole ? t0 : Q.b, Q.b, Q.e, px, py, pz);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            ax[s] += static_cast<double>(px[s]);
+            ay[s] += static_cast<double>(py[s]);
+            az[s] += static_cast<double>(pz[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (ii[s] < se) kick_apply(a, ii[s], ax[s], ay[s], az[s], h);
+}
+
+template <int TEAM, bool SOFT, bool WHOLE>
+__device__ __forceinline__ void kick_small_passes(const EnsArgs &a, int sb, int se, int jb, int je,
+                                                  int tb, double h, double *tile,
+                                                  const Team<TEAM> &tm) {
+    for (int i0 = sb; i0 < se;) {
+        const int S = sinks_per_lane<TEAM>(se - i0, kMaxSpl);
+        if (S >= 4)
+            kick_small_pass<TEAM, 4, SOFT, WHOLE>(a, i0, se, jb, je, tb, h, tile, tm);
+        else if (S == 3)
+            kick_small_pass<TEAM, 3, SOFT, WHOLE>(a, i0, se, jb, je, tb, h, tile, tm);
+        else if (S == 2)
+            kick_small_pass<TEAM, 2, SOFT, WHOLE>(a, i0, se, jb, je, tb, h, tile, tm);
+        else
+            kick_small_pass<TEAM, 1, SOFT, WHOLE>(a, i0, se, jb, je, tb, h, tile, tm);
+        i0 += S * TEAM;
+    }
+}
+
+// Workgroup team (TEAM >= kWG), one kick with the quarter sums: sinks
+// [s0, s1) from L <= kSmallSrc sources [j0, j1) already in the tile (entry
+// j - tb).  With R = waves_per_group > 1 the R waves of each group of 64
+// sinks form 4 / R quarters each and leave them in the partial area (kick
+// slot k); returns whether it did (kick_small_wg_add then adds them after a
+// barrier).  Otherwise every lane sums its sinks' quarters itself.
 template <int TEAM, bool SOFT>
-__device__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, double h,
+__device__ __forceinline__ bool kick_small_wg_quarters(const EnsArgs &a, int k, int s0, int s1,
+                                                       int j0, int j1, int tb, double h,
+                                                       double *tile, float *part,
+                                                       const Team<TEAM> &tm) {
+    constexpr int GM = kMaxSplitGroups<TEAM>;
+    const int G = (s1 - s0 + 63) >> 6, R = waves_per_group<TEAM>(G);
+    if (R == 1) {
+        kick_small_passes<TEAM, SOFT, true>(a, s0, s1, j0, j1, tb, h, tile, tm);
+        return false;
+    }
+    const int w = uni(tm.t >> 6), lane = tm.t & 63;
+    const int per = kQuarters / R, g = w / R, r = w % R;  // per: 1 or 2
+    if (g < G) {
+        const int i = s0 + 64 * g + lane;
+        const int ic = i < s1 ? i : s1 - 1;
+        const double xi[1] = {a.wpos[3 * ic]}, yi[1] = {a.wpos[3 * ic + 1]},
+                     zi[1] = {a.wpos[3 * ic + 2]};
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            if (kk < per) {
+                const Span Q = quarter(j0, j1, r * per + kk);
+                float px[1], py[1], pz[1];
+                kick_span<1, SOFT>(a, xi, yi, zi, tile, tb, Q.b, Q.e, px, py, pz);
+                const int slot = (k * GM + g) * kQuarters + r * per + kk;
+                part[(3 * slot + 0) * 64 + lane] = px[0];
+                part[(3 * slot + 1) * 64 + lane] = py[0];
+                part[(3 * slot + 2) * 64 + lane] = pz[0];
+            }
+        }
+    }
+    return true;
+}
+
+// The quarters of kick slot k added in quarter order (FP64) by each group's
+// first wave, and the velocity update.
+template <int TEAM>
+__device__ __forceinline__ void kick_small_wg_add(const EnsArgs &a, int k, int s0, int s1,
+                                                  double h, const float *part,
+                                                  const Team<TEAM> &tm) {
+    constexpr int GM = kMaxSplitGroups<TEAM>;
+    const int G = (s1 - s0 + 63) >> 6, R = waves_per_group<TEAM>(G);
+    const int w = uni(tm.t >> 6), lane = tm.t & 63;
+    const int g = w / R;
+    if (w % R != 0 || g >= G) return;
+    const int i = s0 + 64 * g + lane;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+    for (int q = 0; q < kQuarters; ++q) {
+        const int slot = (k * GM + g) * kQuarters + q;
+        ax += static_cast<double>(part[(3 * slot + 0) * 64 + lane]);
+        ay += static_cast<double>(part[(3 * slot + 1) * 64 + lane]);
+        az += static_cast<double>(part[(3 * slot + 2) * 64 + lane]);
+    }
+    if (i < s1) kick_apply(a, i, ax, ay, az, h);
+}
+
+// Stage-1 work of a call on [b, e) with <= kSmallSrc particles, fast
+// [b, sb), slow [sb, e), for a workgroup team: the slow particles' first
+// half-drift (src/hold.cuh:123-124, the expression of team_drift) rides on
+// the tile load, each source being loaded by one thread; then slow sinks
+// from every particle and fast sinks from the slow ones (src/hold.cuh:
+// 137-145), both from that tile; both kicks' quarters are added behind one
+// barrier.
+template <int TEAM, bool SOFT>
+__device__ __forceinline__ void team_kicks_small_wg(const EnsArgs &a, int b, int sb, int e,
+                                                    double hd, double h, double *tile,
+                                                    const Team<TEAM> &tm) {
+    float *part = reinterpret_cast<float *>(tile + kTileDoubles);
+    tm.sync();  // every lane is done with the previous tile and partials
+    {
+#pragma clang fp contract(off)
+        const int j = b + tm.t;
+        if (j < e) {
+            double x = a.wpos[3 * j], y = a.wpos[3 * j + 1], z = a.wpos[3 * j + 2];
+            if (j >= sb) {
+                x = x + a.wvel[3 * j] * hd;
+                y = y + a.wvel[3 * j + 1] * hd;
+                z = z + a.wvel[3 * j + 2] * hd;
+                a.wpos[3 * j] = x;
+                a.wpos[3 * j + 1] = y;
+                a.wpos[3 * j + 2] = z;
+            }
+            double *q = tile + kKickStride * tm.t;
+            q[0] = x;
+            q[1] = y;
+            q[2] = z;
+            q[3] = a.wm[j];
+        }
+    }
+    tm.sync();
+    const bool qs = kick_small_wg_quarters<TEAM, SOFT>(a, 0, sb, e, b, e, b, h, tile, part, tm);
+    const bool qf = sb > b &&
+                    kick_small_wg_quarters<TEAM, SOFT>(a, 1, b, sb, sb, e, b, h, tile, part, tm);
+    if (!qs && !qf) return;
+    tm.sync();
+    if (qs) kick_small_wg_add<TEAM>(a, 0, sb, e, h, part, tm);
+    if (qf) kick_small_wg_add<TEAM>(a, 1, b, sb, h, part, tm);
+}
+
+// Sinks [sb, se) from L <= kSmallSrc sources [jb, je) with the quarter
+// sums: a wave team one quarter after the other, a workgroup team from one
+// tile.
+template <int TEAM, bool SOFT>
+__device__ __forceinline__ void team_kick_small(const EnsArgs &a, int sb, int se, int jb, int je,
+                                                double h, double *tile, const Team<TEAM> &tm) {
+    if constexpr (TEAM == 64) {
+        kick_small_passes<64, SOFT, false>(a, sb, se, jb, je, jb, h, tile, tm);
+    } else {
+        float *part = reinterpret_cast<float *>(tile + kTileDoubles);
+        load_kick_tile<TEAM>(a, jb, je, tile, tm);
+        if (kick_small_wg_quarters<TEAM, SOFT>(a, 0, sb, se, jb, je, jb, h, tile, part, tm)) {
+            tm.sync();
+            kick_small_wg_add<TEAM>(a, 0, sb, se, h, part, tm);
+        }
+    }
+}
+
+// vel[i] += h * sum_j m_j (x_j - x_i) / (r^2 + eps2)^{3/2}, sinks [sb, se),
+// sources [jb, je) (absolute indices) in index order through LDS tiles.
+// Up to kSmallSrc sources: the quarter sums above.  More: summed in FP32
+// over blocks of kFold sources from jb (TEAM is a multiple of kFold, so
+// tile edges fall on block edges) folded into a two-sum accumulator.
+template <int TEAM, bool SOFT>
+__device__ __forceinline__ void team_kick(const EnsArgs &a, int sb, int se, int jb, int je, double h,
                           double *tile, const Team<TEAM> &tm) {
     static_assert(TEAM % kFold == 0, "tiles must hold whole fold blocks");
+    static_assert(TEAM <= kWG, "wider teams take team_kicks_small_wg");
     jb = uni(jb);
     je = uni(je);
+    if (je - jb <= kSmallSrc) {
+        if (se > sb && je > jb) team_kick_small<TEAM, SOFT>(a, uni(sb), uni(se), jb, je, h, tile, tm);
+        return;
+    }
     for (int i0 = uni(sb); i0 < se;) {
         const int S = sinks_per_lane<TEAM>(se - i0, kMaxSpl);
         if (S >= 4)
@@ -526,17 +1005,26 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
         } else if (f.stage == 1) {
             const int sb = f.b + f.nf;
             if (f.e > sb) {
-                team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);  // :123-124
-                tm.sync();
-                // :137-145 -- slow sinks from every particle, fast sinks from slow ones
-                team_kick<TEAM, SOFT>(a, o + sb, o + f.e, o + f.b, o + f.e, f.dt, tile, tm);
-                if (f.nf > 0)
-                    team_kick<TEAM, SOFT>(a, o + f.b, o + sb, o + sb, o + f.e, f.dt, tile, tm);
+                // :123-124 and :137-145 -- slow sinks from every particle,
+                // fast sinks from slow ones
+                if constexpr (TEAM > kWG) {
+                    team_kicks_small_wg<TEAM, SOFT>(a, o + f.b, o + sb, o + f.e, f.dt * 0.5, f.dt,
+                                                    tile, tm);
+                } else {
+                    if (TEAM == kWG && f.e - f.b <= kSmallSrc) {
+                        team_kicks_small_wg<TEAM, SOFT>(a, o + f.b, o + sb, o + f.e, f.dt * 0.5,
+                                                        f.dt, tile, tm);
+                    } else {
+                        team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);
+                        tm.sync();
+                        team_kick<TEAM, SOFT>(a, o + sb, o + f.e, o + f.b, o + f.e, f.dt, tile, tm);
+                        if (f.nf > 0)
+                            team_kick<TEAM, SOFT>(a, o + f.b, o + sb, o + sb, o + f.e, f.dt, tile,
+                                                  tm);
+                    }
+                }
                 st.interactions += static_cast<long long>(f.e - sb) * (f.e - f.b) +
                                    static_cast<long long>(f.nf) * (f.e - sb);
-                tm.sync();
-                team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);  // :158-159
-                tm.sync();
             }
             if (tm.t == 0) {
                 sh.stack[sp - 1].stage = 2;
@@ -547,6 +1035,11 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
             }
             tm.sync();
         } else {
+            // :158-159 -- the slow particles' second half-drift, due before
+            // anything outside this call reads them (the fast recursion in
+            // between touches only [b, sb))
+            const int sb = f.b + f.nf;
+            if (f.e > sb) team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);
             if (tm.t == 0) sh.sp = sp - 1;
             tm.sync();
         }
@@ -709,13 +1202,17 @@ __global__ __launch_bounds__(256) void rank_items_kernel(EnsArgs a) {
     a.queue[r] = k < a.n_big ? a.big_list[k] : -1 - (k - a.n_big);
 }
 
+// Three workgroups per CU (<= 168 VGPRs): the batch is throughput-bound,
+// and two per CU measured 5-10 % slower.  flatten inlines the whole engine
+// (both team sizes), which the register allocator needs to stay within
+// that budget without spilling.
 template <bool SOFT>
-__global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
+__global__ __launch_bounds__(kWG, 3) __attribute__((flatten)) void ensemble_kernel(EnsArgs a) {
     __shared__ TeamShared<kWG> sh_wg;
     __shared__ TeamShared<64> sh[kWaves];
     // Source tiles: the whole array for a workgroup team (kWG sources), a
     // quarter per wave team (64 sources each).
-    __shared__ double tiles[kWG * kTsStride];
+    __shared__ double tiles[kTileDoubles + kPartDoubles<kWG>];
     __shared__ int s_item;
     const int w = threadIdx.x >> 6;
     for (;;) {
@@ -748,14 +1245,18 @@ __global__ __launch_bounds__(kWG) void ensemble_kernel(EnsArgs a) {
 // round trips (tile loads, partition, drifts, velocity updates) costs LDS
 // latency instead of L2 latency.  The engine reaches them through generic
 // pointers; the arithmetic and its order are those of the global-memory
-// variant, so both give the same bits.
+// variant, so both give the same bits.  They run on a kSubTeam-thread
+// team, whose every call has <= kSmallSrc sources: each of a call's source
+// quarters gets a wave of its own (four waves per SIMD instead of one).
 constexpr int kSubLds = 256;
+static_assert(kSubLds <= kSmallSrc, "an LDS subtree's calls take the quarter-split path");
 
 template <bool SOFT, bool LDS>
-__global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<double> total, int n,
-                                                      double dt, int calc, SubtreeStats *out) {
-    __shared__ TeamShared<kWG> sh;
-    __shared__ double tile[kWG * kTsStride];
+__global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void subtree_kernel(
+    EnsArgs a, Particles<double> total, int n, double dt, int calc, SubtreeStats *out) {
+    constexpr int TEAM = LDS ? kSubTeam : kWG;
+    __shared__ TeamShared<TEAM> sh;
+    __shared__ double tile[kTileDoubles + kPartDoubles<TEAM>];
     constexpr int C = LDS ? kSubLds : 1;
     __shared__ double l_pos[2][3 * C], l_vel[2][3 * C], l_m[2][C], l_ts[2][C];
     __shared__ int l_id[2][C];
@@ -771,8 +1272,8 @@ __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<doubl
         a.tts = l_ts[1];
         a.tid = l_id[1];
     }
-    Team<kWG> tm;
-    for (int i = tm.t; i < n; i += kWG) {
+    Team<TEAM> tm;
+    for (int i = tm.t; i < n; i += TEAM) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             a.wpos[3 * i + k] = total.pos[3 * i + k];
@@ -782,11 +1283,11 @@ __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<doubl
         a.wts[i] = total.timestep[i];
         a.wid[i] = i;
     }
-    for (int l = tm.t; l < kMaxFrames; l += kWG) sh.level_calls[l] = 0;
+    for (int l = tm.t; l < kMaxFrames; l += TEAM) sh.level_calls[l] = 0;
     tm.sync();
     SysStats st;
-    const int status = team_hold_frame<kWG, SOFT>(a, 0, n, dt, calc, sh, tile, tm, st);
-    for (int g = tm.t; g < n; g += kWG) {
+    const int status = team_hold_frame<TEAM, SOFT>(a, 0, n, dt, calc, sh, tile, tm, st);
+    for (int g = tm.t; g < n; g += TEAM) {
         const int u = a.wid[g];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -804,7 +1305,7 @@ __global__ __launch_bounds__(kWG) void subtree_kernel(EnsArgs a, Particles<doubl
         out->max_level = st.max_level;
         out->status = status;
     }
-    for (int l = tm.t; l < kMaxFrames; l += kWG) out->level_calls[l] = sh.level_calls[l];
+    for (int l = tm.t; l < kMaxFrames; l += TEAM) out->level_calls[l] = sh.level_calls[l];
 }
 
 __global__ void zero_counters_kernel(unsigned int *c) {
@@ -886,7 +1387,8 @@ int launch_subtree(const Particles<double> &total, int n, double dt, int calc, d
     auto kern = static_cast<float>(eps2) > 0.0f
                     ? (lds ? subtree_kernel<true, true> : subtree_kernel<true, false>)
                     : (lds ? subtree_kernel<false, true> : subtree_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kWG), 0, st, a, total, n, dt, calc, out_dev);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, dt, calc,
+                       out_dev);
     HHS_LAUNCH_CHECK("subtree_kernel");
     return HHS_OK;
 }

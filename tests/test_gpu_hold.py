@@ -185,19 +185,21 @@ def test_subtree_fp32_timesteps_match_oracle(hhs, oracle, torch_cuda, n, eps):
     assert np.all(np.abs(ts - ref) <= 1e-6 * ref)
 
 
-@pytest.mark.parametrize("n", [37, 256])
-def test_hold_lds_subtree_bits_equal_ensemble(hhs, torch_cuda, n):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("n", [37, 100, 256])
+def test_hold_lds_subtree_bits_equal_ensemble(hhs, torch_cuda, n, mode):
     """A root step run as one device subtree (n <= 256: the particles live in
-    LDS) gives the same bits as the ensemble engine on the same system
-    (particles in global memory, a one-wave team): the arithmetic and its
-    order do not depend on where the particles are held."""
+    LDS, a 256-thread team that spreads a call's sources over its four waves
+    when it has <= 128 sinks) gives the same bits as the ensemble engine on
+    the same system (particles in global memory, a one-wave team that forms
+    the same source quarters one after the other), in every timestep mode."""
     eps, eta, dt = 1e-3, 0.01, 1.0 / 16
     pos, vel, m = plummer(hhs, n, seed=5150 + n, eps2=eps * eps)
     d = hhs.DeviceParticles.from_host(pos, vel, m)
     # max_level 40 leaves the root call itself inside the subtree's depth range
-    g = hhs.Globals(n, eta, eps, max_level=40, subtree_max=256)
+    g = hhs.Globals(n, eta, eps, max_level=40, subtree_max=256, fast_timesteps=mode)
     g.hold(d, dt, dt)
-    ens = hhs.Ensemble([(pos, vel, m)], dt, dt, eta, eps, max_level=40)
+    ens = hhs.Ensemble([(pos, vel, m)], dt, dt, eta, eps, max_level=40, fast_timesteps=mode)
     ens.run()
     torch_cuda.cuda.synchronize()
     st = g.stats()
