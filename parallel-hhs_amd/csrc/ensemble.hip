@@ -129,8 +129,10 @@ __device__ int team_sum(int x, TeamShared<TEAM> &sh, const Team<TEAM> &tm) {
 }
 
 // Exclusive rank of p among the team's threads in thread order, and the
-// team total.
-template <int TEAM>
+// team total.  TAIL: a barrier after the wave totals are read, so that the
+// next call may overwrite them; a caller that passes another barrier before
+// its next call leaves it out.
+template <int TEAM, bool TAIL = true>
 __device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared<TEAM> &sh,
                                const Team<TEAM> &tm) {
     const int lane = tm.t & 63;
@@ -151,7 +153,7 @@ __device__ void team_scan_bits(bool p, int &excl, int &tot, TeamShared<TEAM> &sh
             before += (k < w) ? c : 0;
             all += c;
         }
-        tm.sync();
+        if constexpr (TAIL) tm.sync();
         excl = before + in_wave;
         tot = all;
     }
@@ -556,10 +558,46 @@ __device__ __forceinline__ void team_timesteps(const EnsArgs &a, int o, int b, i
 
 // Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
 // the fast count after the lone-particle rule (src/split.cuh:321-324).
+//
+// A range that fits the team (e - b <= TEAM) is partitioned in registers:
+// each thread holds its particle's record, one scan gives the fast count
+// and the ranks, and the records are stored at their places after it (the
+// scan's barrier orders every load before every store).  No barrier
+// follows the scan's reads of the wave totals: the caller passes another
+// (the end of the frame's stage) before the next scan.  Larger ranges go
+// through the scratch copy.
 template <int TEAM>
 __device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivot,
                               TeamShared<TEAM> &sh,
                               const Team<TEAM> &tm) {
+    if (e - b <= TEAM) {
+        const int i = o + b + tm.t;
+        const bool in = i < o + e;
+        const int ic = in ? i : o + b;
+        const double x = a.wpos[3 * ic], y = a.wpos[3 * ic + 1], z = a.wpos[3 * ic + 2];
+        const double vx = a.wvel[3 * ic], vy = a.wvel[3 * ic + 1], vz = a.wvel[3 * ic + 2];
+        const double m = a.wm[ic], ts = a.wts[ic];
+        const int id = a.wid[ic];
+        const bool p = in && ts <= pivot;
+        int excl, nf;
+        team_scan_bits<TEAM, false>(p, excl, nf, sh, tm);
+        if (nf == 1) return 0;                  // lone fast particle stays slow, order kept
+        if (nf == 0 || nf == e - b) return nf;  // stable partition is the identity
+        if (in) {
+            const int d = o + b + (p ? excl : nf + tm.t - excl);
+            a.wpos[3 * d] = x;
+            a.wpos[3 * d + 1] = y;
+            a.wpos[3 * d + 2] = z;
+            a.wvel[3 * d] = vx;
+            a.wvel[3 * d + 1] = vy;
+            a.wvel[3 * d + 2] = vz;
+            a.wm[d] = m;
+            a.wts[d] = ts;
+            a.wid[d] = id;
+        }
+        tm.sync();
+        return nf;
+    }
     int c = 0;
     for (int i = o + b + tm.t; i < o + e; i += TEAM) c += (a.wts[i] <= pivot) ? 1 : 0;
     const int nf = team_sum<TEAM>(c, sh, tm);
@@ -962,6 +1000,12 @@ struct SysStats {
 // and the given calc_timestep flag, run to completion with an explicit
 // stack: a root step is (dt_root, calc = 1).  Levels are counted from 0 at
 // this call.  Returns HHS_OK or a status code (uniform).
+//
+// Thread 0 alone writes the stack.  Every thread reads the top frame at the
+// head of an iteration; each stage passes a barrier before thread 0 writes
+// (so every read of this iteration is done) and ends with one after (so
+// the next iteration reads the new state).  A call without fast particles
+// (a leaf) finishes in its stage 1.
 template <int TEAM, bool SOFT>
 __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int calc,
                                TeamShared<TEAM> &sh, double *tile, const Team<TEAM> &tm,
@@ -977,9 +1021,9 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
         if (sp == 0 || sh.status != HHS_OK) break;
         const Frame f = sh.stack[sp - 1];
         const int level = sp - 1;
-        tm.sync();
         if (f.stage == 0) {
             if (level > a.max_level || (a.max_calls > 0 && st.calls >= a.max_calls)) {
+                tm.sync();
                 if (tm.t == 0) sh.status = HHS_ERR_DEPTH;
                 tm.sync();
                 break;
@@ -1025,13 +1069,19 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
                 }
                 st.interactions += static_cast<long long>(f.e - sb) * (f.e - f.b) +
                                    static_cast<long long>(f.nf) * (f.e - sb);
+            } else {
+                tm.sync();
             }
-            if (tm.t == 0) {
+            if (f.nf == 0) {
+                // a leaf: :158-159 (the whole range is slow) once every
+                // kick has read the positions, and the call is done
+                tm.sync();
+                team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);
+                if (tm.t == 0) sh.sp = sp - 1;
+            } else if (tm.t == 0) {
                 sh.stack[sp - 1].stage = 2;
-                if (f.nf > 0) {  // :168-173
-                    sh.stack[sp] = Frame{f.b, sb, 0, 0, f.dt / 2, 1, 0};
-                    sh.sp = sp + 1;
-                }
+                sh.stack[sp] = Frame{f.b, sb, 0, 0, f.dt / 2, 1, 0};  // :168-173
+                sh.sp = sp + 1;
             }
             tm.sync();
         } else {
@@ -1040,6 +1090,7 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
             // between touches only [b, sb))
             const int sb = f.b + f.nf;
             if (f.e > sb) team_drift<TEAM>(a, o, sb, f.e, f.dt * 0.5, tm);
+            tm.sync();
             if (tm.t == 0) sh.sp = sp - 1;
             tm.sync();
         }
