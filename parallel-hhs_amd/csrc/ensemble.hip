@@ -1304,7 +1304,7 @@ static_assert(kSubLds <= kSmallSrc, "an LDS subtree's calls take the quarter-spl
 
 template <bool SOFT, bool LDS>
 __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void subtree_kernel(
-    EnsArgs a, Particles<double> total, int n, double dt, int calc, SubtreeStats *out) {
+    EnsArgs a, Particles<double> total, int n, int clevel, double dt, int calc, SubtreeAcc *acc) {
     constexpr int TEAM = LDS ? kSubTeam : kWG;
     __shared__ TeamShared<TEAM> sh;
     __shared__ double tile[kTileDoubles + kPartDoubles<TEAM>];
@@ -1348,15 +1348,16 @@ __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void
         total.m[u] = a.wm[g];
         total.timestep[u] = a.wts[g];
     }
-    // (out may be host-mapped memory: one store per thread)
+    // the statistics, added to the accumulator (absolute levels)
     if (tm.t == 0) {
-        out->calls = st.calls;
-        out->interactions = st.interactions;
-        out->ts_pairs = st.ts_pairs;
-        out->max_level = st.max_level;
-        out->status = status;
+        acc->calls += st.calls;
+        acc->interactions += st.interactions;
+        acc->ts_pairs += st.ts_pairs;
+        acc->max_level = max(acc->max_level, clevel + st.max_level);
+        if (status != HHS_OK && acc->status == HHS_OK) acc->status = status;
     }
-    for (int l = tm.t; l < kMaxFrames; l += TEAM) out->level_calls[l] = sh.level_calls[l];
+    for (int l = tm.t; l < kMaxFrames && clevel + l < 64; l += TEAM)
+        acc->level_calls[clevel + l] += sh.level_calls[l];
 }
 
 __global__ void zero_counters_kernel(unsigned int *c) {
@@ -1406,11 +1407,11 @@ size_t subtree_workspace_bytes(int n_max) {
     return L.total;
 }
 
-int launch_subtree(const Particles<double> &total, int n, double dt, int calc, double eta,
-                   double eps2, int ts_mode, int max_level, void *ws, size_t ws_bytes,
-                   SubtreeStats *out_dev, hipStream_t st) {
-    HHS_REQUIRE(n > 0 && max_level >= 0 && max_level <= kMaxFrames - 3,
-                "subtree: n=%d, max_level=%d", n, max_level);
+int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
+                   double eta, double eps2, int ts_mode, int max_level, void *ws,
+                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st) {
+    HHS_REQUIRE(n > 0 && clevel >= 0 && max_level >= 0 && max_level <= kMaxFrames - 3 && acc_dev,
+                "subtree: n=%d, clevel=%d, max_level=%d", n, clevel, max_level);
     const EnsLayout L = layout(1, n);
     HHS_REQUIRE(ws && ws_bytes >= L.total, "subtree: workspace of %zu bytes, need %zu",
                 ws_bytes, L.total);
@@ -1438,8 +1439,8 @@ int launch_subtree(const Particles<double> &total, int n, double dt, int calc, d
     auto kern = static_cast<float>(eps2) > 0.0f
                     ? (lds ? subtree_kernel<true, true> : subtree_kernel<true, false>)
                     : (lds ? subtree_kernel<false, true> : subtree_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, dt, calc,
-                       out_dev);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, clevel, dt,
+                       calc, acc_dev);
     HHS_LAUNCH_CHECK("subtree_kernel");
     return HHS_OK;
 }

@@ -154,20 +154,23 @@ template <typename T>
 int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
 
 // ------------------------------------------------- device-side subtrees
-// Statistics of one hold_step subtree run on the device (levels relative to
-// the subtree's first call).
+// Statistics of the device subtrees run since the accumulator was cleared,
+// added up by the subtree kernels themselves (absolute levels; launches on
+// one stream run one after the other, so plain read-modify-writes do).
 constexpr int kSubtreeLevels = 50;
-struct SubtreeStats {
+struct SubtreeAcc {
     long long calls, interactions, ts_pairs;
-    int max_level, status;
-    int level_calls[kSubtreeLevels];
+    int max_level;  // deepest absolute level reached (0 if none)
+    int status;     // HHS_OK, or the status of the first failed subtree
+    long long level_calls[64];
 };
 size_t subtree_workspace_bytes(int n_max);
-// hold_step(total[0..n), dt, calc) with its whole recursion in one
-// workgroup-wide launch (MIXED32 kicks, timestep mode ts_mode); stats go to
-// out_dev.  max_level is relative to this call (<= kSubtreeLevels - 3).
-int launch_subtree(const Particles<double> &total, int n, double dt, int calc, double eta,
-                   double eps2, int ts_mode, int max_level, void *ws, size_t ws_bytes,
-                   SubtreeStats *out_dev, hipStream_t st);
+// hold_step(total[0..n), dt, calc) at level clevel with its whole recursion
+// in one workgroup-wide launch (MIXED32 kicks, timestep mode ts_mode); its
+// statistics are added into acc_dev.  max_level is relative to this call
+// (<= kSubtreeLevels - 3).
+int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
+                   double eta, double eps2, int ts_mode, int max_level, void *ws,
+                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st);
 
 }  // namespace hhs

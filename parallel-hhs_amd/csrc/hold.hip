@@ -61,8 +61,10 @@ struct hhs_globals {
     size_t ts_ws_bytes = 0;
     void *sub_ws = nullptr;
     size_t sub_ws_bytes = 0;
-    // subtree statistics, written by the kernel into host-mapped memory
-    hhs::SubtreeStats *sub_host = nullptr, *sub_map = nullptr;
+    // subtree statistics: added up on the device by the subtree kernels,
+    // fetched once at the end of the hold_step call that launched them
+    hhs::SubtreeAcc *sub_acc = nullptr, *sub_acc_host = nullptr;
+    bool sub_pending = false;
 };
 
 namespace hhs {
@@ -125,26 +127,44 @@ bool use_subtree(const hhs_globals *g, int n, int clevel) {
     return prec == HHS_KICK_MIXED32 && g->max_level - clevel <= kSubtreeLevels - 3;
 }
 
+// The subtree runs in stream order with no host synchronisation: nothing
+// the host recursion decides depends on it, and its statistics (and a
+// failure) are fetched by fetch_subtree_stats at the end of the hold_step
+// call.
 int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
                 bool calc, hipStream_t st) {
-    int rc = launch_subtree(total, n, dt, calc ? 1 : 0, g->eta, g->eps2, g->timestep_mode,
-                            g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_map, st);
+    if (!g->sub_pending) {
+        HHS_HIP(hipMemsetAsync(g->sub_acc, 0, sizeof(SubtreeAcc), st));
+        g->sub_pending = true;
+    }
+    int rc = launch_subtree(total, n, clevel, dt, calc ? 1 : 0, g->eta, g->eps2,
+                            g->timestep_mode, g->max_level - clevel, g->sub_ws, g->sub_ws_bytes,
+                            g->sub_acc, st);
     if (rc) return rc;
+    g->simtime += dt;  // the leaves of a completed subtree tile its dt
+    return HHS_OK;
+}
+
+// Adds the subtrees' statistics to the globals' and reports a subtree that
+// failed (one sync, only when subtrees ran).
+int fetch_subtree_stats(hhs_globals *g, hipStream_t st) {
+    if (!g->sub_pending) return HHS_OK;
+    g->sub_pending = false;
+    HHS_HIP(hipMemcpyAsync(g->sub_acc_host, g->sub_acc, sizeof(SubtreeAcc),
+                           hipMemcpyDeviceToHost, st));
     HHS_HIP(hipStreamSynchronize(st));
-    const SubtreeStats &o = *g->sub_host;
+    const SubtreeAcc &o = *g->sub_acc_host;
     hhs_hold_stats &S = g->stats;
     S.calls += o.calls;
     S.interactions += o.interactions;
     S.ts_pairs += o.ts_pairs;
-    for (int l = 0; l < kSubtreeLevels && clevel + l < 64; ++l)
-        S.calls_per_level[clevel + l] += o.level_calls[l];
-    if (clevel + o.max_level > S.max_level) S.max_level = clevel + o.max_level;
+    for (int l = 0; l < 64; ++l) S.calls_per_level[l] += o.level_calls[l];
+    if (o.max_level > S.max_level) S.max_level = o.max_level;
     if (o.status != HHS_OK) {
         set_error("hold_step: recursion deeper than max_level %d (timesteps collapsed?)",
                   g->max_level);
         return o.status;
     }
-    g->simtime += dt;  // the leaves of a completed subtree tile its dt
     return HHS_OK;
 }
 
@@ -156,12 +176,10 @@ int alloc_subtree(hhs_globals *g, int n_max) {
     if (n_max <= 0) return HHS_OK;
     const int cap = n_max < g->n_total ? n_max : g->n_total;
     const size_t bytes = subtree_workspace_bytes(cap);
-    if (!g->sub_host) {
-        HHS_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->sub_host), sizeof(SubtreeStats),
-                              hipHostMallocMapped | hipHostMallocCoherent));
-        void *map = nullptr;
-        HHS_HIP(hipHostGetDevicePointer(&map, g->sub_host, 0));
-        g->sub_map = static_cast<SubtreeStats *>(map);
+    if (!g->sub_acc) {
+        HHS_HIP(hipMalloc(reinterpret_cast<void **>(&g->sub_acc), sizeof(SubtreeAcc)));
+        HHS_HIP(hipHostMalloc(reinterpret_cast<void **>(&g->sub_acc_host), sizeof(SubtreeAcc),
+                              hipHostMallocDefault));
     }
     HHS_HIP(hipMalloc(&g->sub_ws, bytes));
     g->sub_ws_bytes = bytes;
@@ -326,7 +344,8 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->part_block) (void)hipHostFree(g->part_block);
     if (g->sub_ws) (void)hipFree(g->sub_ws);
     if (g->ts_ws) (void)hipFree(g->ts_ws);
-    if (g->sub_host) (void)hipHostFree(g->sub_host);
+    if (g->sub_acc) (void)hipFree(g->sub_acc);
+    if (g->sub_acc_host) (void)hipHostFree(g->sub_acc_host);
     delete g;
     return HHS_OK;
 }
@@ -395,7 +414,10 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
     HHS_REQUIRE(dt > 0.0, "hold_step: dt must be positive");
     HHS_REQUIRE(total->pos && total->vel && total->m && total->timestep,
                 "hold_step: null field in total");
-    return hold_rec(g, from_c(*total), n, clevel, dt, calc_timestep != 0, as_stream(stream));
+    hipStream_t st = as_stream(stream);
+    const int rc = hold_rec(g, from_c(*total), n, clevel, dt, calc_timestep != 0, st);
+    const int rs = fetch_subtree_stats(g, st);
+    return rc != HHS_OK ? rc : rs;
 }
 
 int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
