@@ -91,6 +91,13 @@ template <typename T> struct KickArgs {
     T h;          // velocity update factor: vel += acc * h   (h = fac * dt)
     double eps2;
     int precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
+    // Optional second group of sinks in the same launch, kicked by segment 0
+    // only (HOLD: the fast sinks from the slow sources beside the slow sinks
+    // from both).  Not in the exact FP64 mode.
+    const T *sb_pos = nullptr;
+    T *sb_vel = nullptr;
+    T *sb_acc = nullptr;
+    int n_sink_b = 0;
 };
 
 struct KickConfig {

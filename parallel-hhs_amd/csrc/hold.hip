@@ -10,8 +10,10 @@
 //    every call and never frees them (src/hold.cuh:56);
 //  * the S<-S and S<-F kicks of the slow particles are one launch over two
 //    source segments (slow, fast), so each slow particle's force is summed
-//    once (src/hold.cuh:137-145); the exact FP64 kick mode keeps the two
-//    launches so its sums follow the CPU order;
+//    once (src/hold.cuh:137-145), and the F<-S kick of the fast particles
+//    rides in the same launch as a second group of sinks; the exact FP64
+//    kick mode keeps the reference's three kicks so its sums follow the
+//    CPU order;
 //  * the only host synchronisation per call is the count read after the
 //    split (the reference blocks at the same point, src/split.cuh:430); the
 //    split writes the counts straight into host-mapped memory;
@@ -238,26 +240,31 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
         // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow.
         // In the exact FP64 mode the S<-S and S<-F kicks are two velocity
         // updates, as in the reference, so the sums match the CPU order.
+        // Otherwise one launch kicks both: the fast sinks ride along as the
+        // kick's second sink group, fed by the slow segment only.
         const bool exact = g->kick_precision == HHS_KICK_F64_EXACT;
         KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
                             lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
                             lb.fast.m,   exact ? 0 : nf, dt,      g->eps2,
                             g->kick_precision};
+        if (!exact && nf > 0) {
+            ks.sb_pos = lb.fast.pos;
+            ks.sb_vel = lb.fast.vel;
+            ks.sb_acc = lb.fast.acc;
+            ks.n_sink_b = nf;
+        }
         if ((rc = launch_kick<double>(ks, st))) return rc;
         if (exact && nf > 0) {
             KickArgs<double> ksf{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,      lb.fast.pos,
                                  lb.fast.m,   nf,          nullptr,     nullptr, 0,
                                  dt,          g->eps2,     g->kick_precision};
             if ((rc = launch_kick<double>(ksf, st))) return rc;
-        }
-        S.interactions += static_cast<long long>(ns) * (ns + nf);
-        if (nf > 0) {
             KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
                                 lb.slow.m,   ns,          nullptr,     nullptr, 0,
                                 dt,          g->eps2,     g->kick_precision};
             if ((rc = launch_kick<double>(kf, st))) return rc;
-            S.interactions += static_cast<long long>(nf) * ns;
         }
+        S.interactions += static_cast<long long>(ns) * (ns + nf) + static_cast<long long>(nf) * ns;
         // src/hold.cuh:158-159: done by the scatter-back below
     }
     // src/hold.cuh:168-173

@@ -171,18 +171,30 @@ template <int S, bool SOFT> struct KickAcc<double, S, 3, SOFT> {
 // waves splitting the chunk (segment 0 then segment 1), per-wave totals
 // merged through LDS in wave order (deterministic).  With K == 1 the
 // epilogue writes acc <- a, vel += a * h; otherwise the chunk's FP64 partial
-// sums go to partial[k][i][3] for kick_reduce_kernel.
+// sums go to partial[k][i][3] for kick_reduce_kernel.  Tiles past the first
+// group's belong to the optional second group (sinks gb_*, segment 0 only);
+// its partials follow the first group's (i + n_sink).
 template <typename T, int S, int W, int P, bool SOFT>
 __global__ __launch_bounds__(64 * W) void kick_kernel(
-    const T *__restrict__ si_pos, T *__restrict__ si_vel, T *__restrict__ si_acc, int n_sink,
+    const T *__restrict__ ga_pos, T *__restrict__ ga_vel, T *__restrict__ ga_acc, int n_sink,
+    const T *__restrict__ gb_pos, T *__restrict__ gb_vel, T *__restrict__ gb_acc, int n_sink_b,
     const T *__restrict__ s0_pos, const T *__restrict__ s0_m, int n0,
-    const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1, T h, float eps2f,
+    const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1_a, T h, float eps2f,
     double eps2, int K, double *__restrict__ partial, const float *__restrict__ mf) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * 3 * S * 64];
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile0 = blockIdx.x * (64 * S);
+    const int tiles_a = (n_sink + 64 * S - 1) / (64 * S);
+    const bool grp_b = static_cast<int>(blockIdx.x) >= tiles_a;
+    const T *si_pos = grp_b ? gb_pos : ga_pos;
+    T *si_vel = grp_b ? gb_vel : ga_vel;
+    T *si_acc = grp_b ? gb_acc : ga_acc;
+    const int n1 = grp_b ? 0 : n1_a;
+    const int pofs = grp_b ? n_sink : 0;  // partial row of this group's first sink
+    const int n_rows = n_sink + n_sink_b;
+    if (grp_b) n_sink = n_sink_b;
+    const int tile0 = (static_cast<int>(blockIdx.x) - (grp_b ? tiles_a : 0)) * (64 * S);
 
     T xi[S], yi[S], zi[S];
 #pragma unroll
@@ -235,7 +247,7 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
         }
     }
     if (partial) {
-        double *q = partial + 3 * static_cast<size_t>(blockIdx.y) * n_sink;
+        double *q = partial + 3 * (static_cast<size_t>(blockIdx.y) * n_rows + pofs);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int i = tile0 + s * 64 + lane;
@@ -288,22 +300,29 @@ __global__ __launch_bounds__(256) void masses_to_f32_kernel(const T *__restrict_
 }
 
 // Sum of the K chunk partials of each sink in chunk order (deterministic),
-// then the epilogue of kick_kernel.
+// then the epilogue of kick_kernel; rows past n_sink are the second group's.
 template <typename T>
-__global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ si_vel,
-                                                          T *__restrict__ si_acc, int n_sink,
+__global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ ga_vel,
+                                                          T *__restrict__ ga_acc, int n_sink,
+                                                          T *__restrict__ gb_vel,
+                                                          T *__restrict__ gb_acc, int n_sink_b,
                                                           const double *__restrict__ partial,
                                                           int K, T h) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_sink) return;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int n_rows = n_sink + n_sink_b;
+    if (r >= n_rows) return;
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int k = 0; k < K; ++k) {
-        const double *q = partial + 3 * (static_cast<size_t>(k) * n_sink + i);
+        const double *q = partial + 3 * (static_cast<size_t>(k) * n_rows + r);
         ax += q[0];
         ay += q[1];
         az += q[2];
     }
+    const bool grp_b = r >= n_sink;
+    T *si_vel = grp_b ? gb_vel : ga_vel;
+    T *si_acc = grp_b ? gb_acc : ga_acc;
+    const int i = grp_b ? r - n_sink : r;
     const T a0 = static_cast<T>(ax), a1 = static_cast<T>(ay), a2 = static_cast<T>(az);
     if (si_acc) {
         si_acc[3 * i + 0] = a0;
@@ -318,22 +337,25 @@ __global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ si_vel
 template <typename T, int S, int W, int P>
 int launch_swp(const KickArgs<T> &a, int K, double *partial, const float *mf, hipStream_t st) {
     const int tile = 64 * S;
-    dim3 grid((a.n_sink + tile - 1) / tile, K), block(64 * W);
+    dim3 grid((a.n_sink + tile - 1) / tile + (a.n_sink_b + tile - 1) / tile, K), block(64 * W);
     const double eps2 = a.eps2;
     const float eps2f = static_cast<float>(a.eps2);
     double *part = K > 1 ? partial : nullptr;
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, true>), grid, block, 0, st, a.si_pos,
-                           a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2, K, part, mf);
+                           a.si_vel, a.si_acc, a.n_sink, a.sb_pos, a.sb_vel, a.sb_acc, a.n_sink_b,
+                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K,
+                           part, mf);
     else
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, false>), grid, block, 0, st, a.si_pos,
-                           a.si_vel, a.si_acc, a.n_sink, a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m,
-                           a.n1, a.h, eps2f, eps2, K, part, mf);
+                           a.si_vel, a.si_acc, a.n_sink, a.sb_pos, a.sb_vel, a.sb_acc, a.n_sink_b,
+                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K,
+                           part, mf);
     HHS_LAUNCH_CHECK("kick_kernel");
     if (K > 1) {
-        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((a.n_sink + 255) / 256), dim3(256), 0, st,
-                           a.si_vel, a.si_acc, a.n_sink, part, K, a.h);
+        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((a.n_sink + a.n_sink_b + 255) / 256),
+                           dim3(256), 0, st, a.si_vel, a.si_acc, a.n_sink, a.sb_vel, a.sb_acc,
+                           a.n_sink_b, part, K, a.h);
         HHS_LAUNCH_CHECK("kick_reduce_kernel");
     }
     return HHS_OK;
@@ -420,14 +442,22 @@ int pick_split(int n_sink, int n_src, int S, int W) {
 }
 
 template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
-    HHS_REQUIRE(a.n_sink >= 0 && a.n0 >= 0 && a.n1 >= 0, "kick: negative size");
-    if (a.n_sink == 0) return HHS_OK;
-    HHS_REQUIRE(a.si_pos && a.si_vel, "kick: null sink pos/vel");
+    HHS_REQUIRE(a.n_sink >= 0 && a.n_sink_b >= 0 && a.n0 >= 0 && a.n1 >= 0,
+                "kick: negative size");
+    const int n_rows = a.n_sink + a.n_sink_b;  // sinks of both groups
+    if (n_rows == 0) return HHS_OK;
+    HHS_REQUIRE(a.n_sink == 0 || (a.si_pos && a.si_vel), "kick: null sink pos/vel");
+    HHS_REQUIRE(a.n_sink_b == 0 || (a.sb_pos && a.sb_vel), "kick: null second-group pos/vel");
     HHS_REQUIRE(a.n0 == 0 || (a.s0_pos && a.s0_m), "kick: null source segment 0");
     HHS_REQUIRE(a.n1 == 0 || (a.s1_pos && a.s1_m), "kick: null source segment 1");
-    KickConfig c = kick_pick_config(a.n_sink, a.n0 + a.n1);
+    // the first group's sources (the most any sink sees) set the waves per
+    // group and the split; the second group's waves take shorter chunks
+    const int n_src_cfg = a.n_sink > 0 ? a.n0 + a.n1 : a.n0;
+    KickConfig c = kick_pick_config(n_rows, n_src_cfg);
     const int P = a.precision >= 0 ? a.precision : g_precision.load();
     HHS_REQUIRE(P >= HHS_KICK_MIXED32 && P <= HHS_KICK_F64_EXACT, "kick: precision mode %d", P);
+    HHS_REQUIRE(a.n_sink_b == 0 || P != HHS_KICK_F64_EXACT,
+                "kick: the exact FP64 mode takes one sink group per launch");
     if (P == HHS_KICK_F64_EXACT && std::is_same<T, double>::value) {
         // One lane per sink, one wave per group: each sink sums its sources
         // in index order.  Segment 1 follows segment 0 in the same sum.
@@ -435,13 +465,13 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
         c.waves = 1;
     }
     // The exact mode keeps one sequential sum per sink.
-    const int K = (P == HHS_KICK_F64_EXACT) ? 1 : pick_split(a.n_sink, a.n0 + a.n1,
+    const int K = (P == HHS_KICK_F64_EXACT) ? 1 : pick_split(n_rows, n_src_cfg,
                                                              c.sinks_per_lane, c.waves);
     static thread_local ThreadScratch scratch, mass_scratch;
     double *partial = nullptr;
     if (K > 1) {
         void *p = nullptr;
-        int rc = scratch.acquire(24 * static_cast<size_t>(a.n_sink) * K, st, &p);
+        int rc = scratch.acquire(24 * static_cast<size_t>(n_rows) * K, st, &p);
         if (rc) return rc;
         partial = static_cast<double *>(p);
     }
@@ -451,8 +481,10 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     // extra launch costs more than it saves.
     float *mf = nullptr;
     const int n_src = a.n0 + a.n1;
+    const double work = static_cast<double>(a.n_sink) * n_src +
+                        static_cast<double>(a.n_sink_b) * a.n0;
     const bool conv = std::is_same<T, double>::value && P == HHS_KICK_MIXED32 &&
-                      static_cast<double>(a.n_sink) * n_src >= kConvMinWork;
+                      work >= kConvMinWork;
     if (conv) {
         void *p = nullptr;
         int rc = mass_scratch.acquire(4 * static_cast<size_t>(n_src), st, &p);
