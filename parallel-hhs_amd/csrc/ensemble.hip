@@ -291,10 +291,11 @@ template <int TEAM> __device__ __forceinline__ int sinks_per_lane(int rem, int c
 // kQuarters quarters of ceil(L / kQuarters); each quarter's minimum (a
 // fresh fold from 1e30) or force (a plain FP32 sum from zero) is formed on
 // its own, and the quarters are combined in quarter order (a fold of the
-// fractions; FP64 adds of the sums).  A workgroup team with <= 128 sinks
-// forms the quarters in parallel, one or two per wave, and combines them
-// through LDS; other teams form them one after the other.  The bits are
-// the same either way.
+// fractions; FP64 adds of the sums).  A workgroup team with at least two
+// waves per group of 64 sinks (<= 128 sinks on kWG threads, <= 256 on the
+// kSubTeam threads of an LDS subtree) forms the quarters in parallel, one
+// or two per wave, and combines them through LDS; other teams form them
+// one after the other.  The bits are the same either way.
 constexpr int kQuarters = 4;
 constexpr int kSmallSrc = kWG;  // the whole source range fits one workgroup tile
 // A workgroup team's quarter partials sit in its LDS tile array after the
