@@ -508,6 +508,8 @@ def main():
                     help="skip the all-threads, cpu_split and N=1024 CPU HOLD timings")
     ap.add_argument("--loop-sample", type=int, default=12,
                     help="systems timed through the plain per-system loop (0 skips it)")
+    ap.add_argument("--fp64-sample", type=int, default=6,
+                    help="ensemble systems missing |dE/E| <= 1e-6 rerun by the FP64-only HOLD")
     ap.add_argument("--serve-full", action="store_true",
                     help="served stream over the full N range [64, 65536] (long)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -617,7 +619,8 @@ def main():
         out["hold_n16384"] = hold16k_bench(hhs, torch)
         log(f"hold n=16384: {out['hold_n16384']}")
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample)
+        out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample,
+                                         fp64_sample=args.fp64_sample)
         log(f"ensemble: {out['ensemble']}")
     if not args.no_serve:
         if args.serve_full:
