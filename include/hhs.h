@@ -269,7 +269,9 @@ int hhs_update_f32(hhs_particles_f32 *prev, const hhs_particles_f32 *cur, int n,
 /* FP64 kinetic and potential energy, E = sum 1/2 m v^2 -
  * sum_{i<j} m_i m_j / sqrt(r^2 + eps2) (no reference counterpart; the
  * diagnostic required for |dE/E|).  Deterministic for a given n.
- * Blocks the calling thread until the result is on the host. */
+ * Blocks the calling thread until the result is on the host.  Returns
+ * HHS_ERR_NONFINITE (with both sums still written) when either sum is NaN
+ * or infinite, i.e. when the state holds a NaN or Inf. */
 int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n,
                    double eps2, double *kinetic, double *potential, void *stream);
 

@@ -85,7 +85,24 @@ struct EnsArgs {
     hhs_system_result *results;
     int *done;
     int ts_mode;
+    // Decision guard of the approximate timestep modes (pair_math.h): the
+    // band of every call's TsGuard; guard == 0 in the exact mode.
+    double guard_qmax;
+    unsigned long long guard_lo, guard_hi;
+    int guard;
 };
+
+// The guard's band for a timestep mode (none for the exact mode).
+inline void set_guard(EnsArgs &a, int ts_mode) {
+    const double band = ts_mode == HHS_TIMESTEP_FP32   ? kGuardBandFp32
+                        : ts_mode == HHS_TIMESTEP_FAST ? kGuardBandFast
+                                                       : 0.0;
+    const TsGuard g = make_ts_guard(1.0, band);
+    a.guard = band > 0.0 ? 1 : 0;
+    a.guard_qmax = g.qmax;
+    a.guard_lo = g.lo;
+    a.guard_hi = g.hi;
+}
 
 // A workgroup team (TEAM = the block size) or a one-wave team (TEAM = 64).
 template <int TEAM> struct Team {
@@ -557,6 +574,44 @@ __device__ __forceinline__ void team_timesteps(const EnsArgs &a, int o, int b, i
         team_timesteps_mode<TEAM, HHS_TIMESTEP_EXACT>(a, jb, je, tile, tm);
 }
 
+// Decision guard (pair_math.h) for the call [jb, je) with time step dt:
+// every timestep within the band of a pivot dt * 2^-k is recomputed with
+// the exact formula over the call's sources, by the wave of the thread that
+// owns the sink (sink i belongs to thread (i - jb) mod TEAM, as in the
+// timestep passes and the partition), 64 sources per step and a wave
+// minimum.  The owner writes the value, so the partition, which reads sink
+// i on that same thread, needs no barrier; a minimum does not depend on
+// order, so the value is the exact mode's bit for bit.
+template <int TEAM>
+__device__ void team_ts_guard(const EnsArgs &a, int jb, int je, double dt, const Team<TEAM> &tm) {
+    const TsGuard g{dt, a.guard_qmax, a.guard_lo, a.guard_hi};
+    const int lane = tm.t & 63;
+    for (int i0 = jb; i0 < je; i0 += TEAM) {
+        const int i = i0 + tm.t;
+        uint64_t mask = __ballot(i < je && guard_near_pivot(g, a.wts[i]));
+        while (mask) {
+            const int l = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+            mask &= mask - 1;
+            const int s = __shfl(i, l, 64);
+            const double xi = a.wpos[3 * s], yi = a.wpos[3 * s + 1], zi = a.wpos[3 * s + 2];
+            const double vxi = a.wvel[3 * s], vyi = a.wvel[3 * s + 1], vzi = a.wvel[3 * s + 2];
+            const double mi = a.wm[s];
+            double best = TsConst<double>::huge;
+            for (int j = jb + lane; j < je; j += 64)
+                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
+                                      a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
+                                      a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta,
+                                      a.c_ff, a.eps2, j != s, best);
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const double o = __shfl_xor(best, d, 64);
+                best = o < best ? o : best;
+            }
+            if (lane == l) a.wts[s] = best;
+        }
+    }
+}
+
 // Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
 // the fast count after the lone-particle rule (src/split.cuh:321-324).
 //
@@ -631,6 +686,16 @@ __device__ void team_drift(const EnsArgs &a, int o, int b, int e, double h,
         a.wpos[k] = a.wpos[k] + a.wvel[k] * h;  // src/drift.cuh:25
 }
 
+// vel[i] += a * h, the same expression on every path, with the FP32 force
+// sum a debiased for v_rsq_f32's mean error (kRsqDebias).
+__device__ __forceinline__ void kick_apply(const EnsArgs &a, int i, double ax, double ay,
+                                           double az, double h) {
+    const double hc = h * kRsqDebias;
+    a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * hc;
+    a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * hc;
+    a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * hc;
+}
+
 // One pass of team_kick for S sinks per lane (sinks i0 + t + s * TEAM); the
 // per-sink sum is the same for every S.
 template <int TEAM, int S, bool SOFT>
@@ -689,9 +754,7 @@ __device__ void team_kick_pass(const EnsArgs &a, int i0, int se, int jb, int je,
             const double ax = static_cast<double>(hx[s]) + static_cast<double>(lx[s]);
             const double ay = static_cast<double>(hy[s]) + static_cast<double>(ly[s]);
             const double az = static_cast<double>(hz[s]) + static_cast<double>(lz[s]);
-            a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
-            a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
-            a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
+            kick_apply(a, i, ax, ay, az, h);
         }
     }
 }
@@ -712,13 +775,6 @@ __device__ __forceinline__ void load_kick_tile(const EnsArgs &a, int s0, int je,
     tm.sync();
 }
 
-// vel[i] += a * h, the same expression on every path.
-__device__ __forceinline__ void kick_apply(const EnsArgs &a, int i, double ax, double ay,
-                                           double az, double h) {
-    a.wvel[3 * i + 0] = a.wvel[3 * i + 0] + ax * h;
-    a.wvel[3 * i + 1] = a.wvel[3 * i + 1] + ay * h;
-    a.wvel[3 * i + 2] = a.wvel[3 * i + 2] + az * h;
-}
 
 // Plain FP32 force sums from zero over sources [qb, qe) (tile entry j - t0)
 // for S sinks.
@@ -1036,6 +1092,7 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
                 team_timesteps<TEAM>(a, o, f.b, f.e, tile, tm);
                 st.ts_pairs += static_cast<long long>(f.e - f.b) * (f.e - f.b - 1);
                 tm.sync();
+                if (a.guard) team_ts_guard<TEAM>(a, o + f.b, o + f.e, f.dt, tm);
             }
             const int nf = team_partition<TEAM>(a, o, f.b, f.e, f.dt, sh, tm);  // :79-83
             if (tm.t == 0) {
@@ -1435,6 +1492,7 @@ int launch_subtree(const Particles<double> &total, int n, int clevel, double dt,
     a.tts = reinterpret_cast<double *>(w + L.tts);
     a.tid = reinterpret_cast<int *>(w + L.tid);
     a.ts_mode = ts_mode;
+    set_guard(a, ts_mode);
     const bool lds = n <= kSubLds;
     // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula
     auto kern = static_cast<float>(eps2) > 0.0f
@@ -1540,6 +1598,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
                     d->timestep_mode == HHS_TIMESTEP_FP32,
                 "ensemble: bad timestep_mode %d", d->timestep_mode);
     a.ts_mode = d->timestep_mode;
+    set_guard(a, a.ts_mode);
 
     // Persistent grid: enough workgroups for every big system and every
     // group of four small ones, capped at 8 per CU.

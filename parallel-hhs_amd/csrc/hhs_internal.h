@@ -147,10 +147,13 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
 // HOLD-internal fused forms (hold.hip): the split that also half-drifts the
 // slow set and reports the raw fast count, its lone-particle follow-up, and
 // the scatter-back of both sets with the second slow half-drift.
+// guard_band > 0 runs the decision guard of the approximate timestep modes
+// (pair_math.h) on the set's timesteps before the predicates are formed.
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
-                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st);
+                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st,
+                      double guard_band = 0.0, double eta = 0.0, double eps2 = 0.0);
 template <typename T>
 int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
                            hipStream_t st);

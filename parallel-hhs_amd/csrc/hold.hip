@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "hhs_internal.h"
+#include "pair_math.h"
 
 namespace hhs {
 
@@ -55,6 +56,7 @@ struct hhs_globals {
     hhs_hold_stats stats{};
     int timestep_mode = HHS_TIMESTEP_EXACT;
     int kick_precision = -1;  // HHS_KICK_*; -1 = the process-wide setting
+    int active_kick = HHS_KICK_MIXED32;  // the mode resolved at hold_step entry
     long long root_steps = 0; // root steps completed by hhs_evolve_f64
     // Calls with at most subtree_max particles run their whole subtree in one
     // device launch (0 = never).
@@ -121,12 +123,26 @@ int ensure_level(hhs_globals *g, int level) {
 // because one workgroup then does work the whole GPU would do faster.
 constexpr int kDefaultSubtreeMax = 256;
 
+// The kick mode of one hold_step call: the globals' own, or the
+// process-wide setting when that is -1, read once at entry so that every
+// level of the recursion uses the same arithmetic.
+int resolve_kick(const hhs_globals *g) {
+    return g->kick_precision >= 0 ? g->kick_precision : hhs_kick_get_precision();
+}
+
 // The device subtree runs the MIXED32 kick; other kick modes keep the host
 // recursion so that their arithmetic is the one requested.
 bool use_subtree(const hhs_globals *g, int n, int clevel) {
     if (g->subtree_max <= 0 || n > g->subtree_max || !g->sub_ws) return false;
-    const int prec = g->kick_precision >= 0 ? g->kick_precision : hhs_kick_get_precision();
-    return prec == HHS_KICK_MIXED32 && g->max_level - clevel <= kSubtreeLevels - 3;
+    return g->active_kick == HHS_KICK_MIXED32 && g->max_level - clevel <= kSubtreeLevels - 3;
+}
+
+// Band of the split's decision guard after a timestep evaluation in the
+// globals' mode (pair_math.h): approximate modes only.
+double guard_band(const hhs_globals *g) {
+    return g->timestep_mode == HHS_TIMESTEP_FP32   ? kGuardBandFp32
+           : g->timestep_mode == HHS_TIMESTEP_FAST ? kGuardBandFast
+                                                   : 0.0;
 }
 
 // The subtree runs in stream order with no host synchronisation: nothing
@@ -220,8 +236,11 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     const double h = dt * 0.5;  // the half-drift of src/hold.cuh:123-124,158-159
     // src/hold.cuh:79-83 with the first slow half-drift, then the count
     // read-back (src/split.cuh:430)
+    // with the decision guard after a timestep evaluation in an approximate
+    // mode, so that the slow/fast decisions are the exact mode's
     rc = launch_split_hold<double>(total, lb.slow, lb.fast, dt, n, g->part_map + clevel,
-                                   g->raw_map + clevel, h, g->scan_ws, g->scan_ws_bytes, st);
+                                   g->raw_map + clevel, h, g->scan_ws, g->scan_ws_bytes, st,
+                                   calc ? guard_band(g) : 0.0, g->eta, g->eps2);
     if (rc) return rc;
     HHS_HIP(hipStreamSynchronize(st));
     const int ns = g->part_host[clevel].N_slow;
@@ -242,11 +261,12 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
         // updates, as in the reference, so the sums match the CPU order.
         // Otherwise one launch kicks both: the fast sinks ride along as the
         // kick's second sink group, fed by the slow segment only.
-        const bool exact = g->kick_precision == HHS_KICK_F64_EXACT;
+        const int prec = g->active_kick;
+        const bool exact = prec == HHS_KICK_F64_EXACT;
         KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
                             lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
                             lb.fast.m,   exact ? 0 : nf, dt,      g->eps2,
-                            g->kick_precision};
+                            prec};
         if (!exact && nf > 0) {
             ks.sb_pos = lb.fast.pos;
             ks.sb_vel = lb.fast.vel;
@@ -257,11 +277,11 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
         if (exact && nf > 0) {
             KickArgs<double> ksf{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,      lb.fast.pos,
                                  lb.fast.m,   nf,          nullptr,     nullptr, 0,
-                                 dt,          g->eps2,     g->kick_precision};
+                                 dt,          g->eps2,     prec};
             if ((rc = launch_kick<double>(ksf, st))) return rc;
             KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
                                 lb.slow.m,   ns,          nullptr,     nullptr, 0,
-                                dt,          g->eps2,     g->kick_precision};
+                                dt,          g->eps2,     prec};
             if ((rc = launch_kick<double>(kf, st))) return rc;
         }
         S.interactions += static_cast<long long>(ns) * (ns + nf) + static_cast<long long>(nf) * ns;
@@ -422,6 +442,7 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
     HHS_REQUIRE(total->pos && total->vel && total->m && total->timestep,
                 "hold_step: null field in total");
     hipStream_t st = as_stream(stream);
+    g->active_kick = resolve_kick(g);
     const int rc = hold_rec(g, from_c(*total), n, clevel, dt, calc_timestep != 0, st);
     const int rs = fetch_subtree_stats(g, st);
     return rc != HHS_OK ? rc : rs;

@@ -88,10 +88,11 @@ template <typename T, int S, bool SOFT> struct KickAcc<T, S, 0, SOFT> {
             }
         }
     }
+    // debiased for v_rsq_f32's mean error (kRsqDebias)
     __device__ void total(int s, double &ax, double &ay, double &az) const {
-        ax = static_cast<double>(hx[s]) + static_cast<double>(lx[s]);
-        ay = static_cast<double>(hy[s]) + static_cast<double>(ly[s]);
-        az = static_cast<double>(hz[s]) + static_cast<double>(lz[s]);
+        ax = (static_cast<double>(hx[s]) + static_cast<double>(lx[s])) * kRsqDebias;
+        ay = (static_cast<double>(hy[s]) + static_cast<double>(ly[s])) * kRsqDebias;
+        az = (static_cast<double>(hz[s]) + static_cast<double>(lz[s])) * kRsqDebias;
     }
 };
 
@@ -128,10 +129,12 @@ template <int S, int P, bool SOFT> struct KickAcc64 {
             }
         }
     }
+    // P = 1 is debiased for v_rsq_f32's mean error (kRsqDebias)
     __device__ void total(int s, double &x, double &y, double &z) const {
-        x = ax[s];
-        y = ay[s];
-        z = az[s];
+        const double c = P == 1 ? kRsqDebias : 1.0;
+        x = ax[s] * c;
+        y = ay[s] * c;
+        z = az[s] * c;
     }
 };
 

@@ -194,6 +194,48 @@ __device__ __forceinline__ float pair_tau_f32(double xi, double yi, double zi, d
     return __builtin_fminf(__builtin_fminf(t_ff, t_fb), TsConst<float>::huge);
 }
 
+// ------------------------------------------------------ decision guard
+// HOLD compares each timestep with the pivots dt * 2^-k (k >= 0) of the call
+// that computed it and of every level below (src/split.cuh:115,
+// src/hold.cuh:79-83,116-121).  An approximate timestep (FP32 or FAST mode)
+// lands on the other side of a pivot from the exact one only when the two
+// straddle it, i.e. when the approximate value lies within its error bound
+// of the pivot.  The guard recomputes exactly such timesteps -- those
+// within `band` (relative) of a pivot -- with the exact formula over the
+// same sources, so the slow/fast decisions of the approximate modes are
+// those of the exact mode.  band: FP32 1e-5 (the FP32 criterion is within
+// 1e-6 relative), FAST 1e-12 (within 1e-14).  About 2 band / ln 2 of the
+// timesteps (3e-5 in FP32) are recomputed.
+constexpr double kGuardBandFp32 = 1e-5;
+constexpr double kGuardBandFast = 1e-12;
+
+struct TsGuard {
+    double dt;                   // the call's dt (the largest pivot)
+    double qmax;                 // 1 + band
+    unsigned long long lo, hi;   // band in units of the FP64 mantissa
+};
+
+inline TsGuard make_ts_guard(double dt, double band) {
+    TsGuard g;
+    g.dt = dt;
+    g.qmax = 1.0 + band;
+    g.lo = static_cast<unsigned long long>(band * 4503599627370496.0);            // band * 2^52
+    g.hi = static_cast<unsigned long long>((1.0 - 2.0 * band) * 4503599627370496.0);
+    return g;
+}
+
+// True when ts / dt lies within the band of a power of two 2^-k, k >= 0:
+// q = 2^e (1 + f) is within band of 2^e when f <= band and within band of
+// 2^(e+1) when (1 - f) / 2 <= band.  NaN, zero and values above every pivot
+// are never flagged.
+__device__ __forceinline__ bool guard_near_pivot(const TsGuard &g, double ts) {
+    const double q = ts / g.dt;
+    if (!(q > 0.0) || q > g.qmax) return false;
+    const unsigned long long f =
+        static_cast<unsigned long long>(__double_as_longlong(q)) & 0xFFFFFFFFFFFFFull;
+    return f <= g.lo || f >= g.hi;
+}
+
 // Knuth two-sum of hi + p; the rounding error is carried into lo.
 __device__ __forceinline__ void two_sum_add(float &hi, float &lo, float p) {
     const float s = hi + p;
@@ -225,6 +267,18 @@ __device__ __forceinline__ void pair_accel_exact(double xj, double yj, double zj
     ay = skip ? ay : ay + dy * s;
     az = skip ? az : az + dz * s;
 }
+
+// Mean bias of v_rsq_f32.  Its relative error has a nonzero mean: -7.371e-9
+// over inputs spread log-uniformly over whole octaves (MI355X, 2^22 inputs
+// each in [1, 4), [0.5, 2) and [1e-8, 1): -7.371e-9, -7.371e-9, -7.342e-9;
+// rms 2.79e-8, max 9.4e-8).  The pair weight m * rsq^3 therefore runs
+// 3 * 7.371e-9 = 2.211e-8 low on average, and a direct sum over many
+// sources turns that into a systematic force-scale error (mean relative
+// error along a of -2.2e-8 on the N = 16384 and 131072 spheres, most of the
+// rms error of 3e-8) -- an energy drift that FP64 forces do not have.  The
+// force paths that use v_rsq_f32 multiply their FP64 totals by kRsqDebias,
+// which removes the mean and leaves the zero-mean part.
+constexpr double kRsqDebias = 1.0 + 2.2113e-8;
 
 // One source's contribution to a sink's acceleration (FP32 from FP64-derived
 // separations); SOFT = (eps2 > 0), else pairs with r^2 == 0 are skipped

@@ -13,8 +13,10 @@
 //  * large systems run the per-system HOLD driver (hhs_hold_f64) on one of
 //    `big_workers` worker threads, each with its own stream and a reusable
 //    hhs_globals sized for max_n.
-// All streams are non-blocking and at the default priority; device buffers
-// of a batch come from the stream-ordered allocator, so no call forces a
+// All streams are non-blocking and at the default priority; each batch
+// stream keeps grow-only device (stream-ordered) and host-mapped buffers,
+// so batches allocate nothing once the buffers have grown and nothing is
+// allocated or freed while the scheduler's lock is held; no call forces a
 // device-wide synchronisation while other streams are busy.  A request's
 // result depends only on its own data (the path is chosen by n), so it is
 // the same alone and in any mix.
@@ -48,14 +50,29 @@ struct Request {
     hhs_request_info info{};
 };
 
+// One batch stream with its buffers.  The buffers only grow and are
+// reused by every batch of the slot: the device block (inputs, workspace)
+// through the stream-ordered allocator on the slot's stream, the
+// host-mapped block (pos/vel in and out, results, flags) geometrically,
+// keeping outgrown host blocks until the server is destroyed -- so no batch
+// frees host memory, whose free would synchronise the whole device.  Only
+// launch_batch (no lock held) allocates; a slot's next batch starts after
+// the previous one has completed and been handed back.
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    void *dev = nullptr;
+    size_t dev_bytes = 0;
+    void *host = nullptr, *host_dev = nullptr;  // host address / device address
+    size_t host_bytes = 0;
+    std::vector<void *> retired_host;
+};
+
 struct Batch {
     std::vector<Request *> reqs;
     std::vector<size_t> first;    // particle offset of each request
     std::vector<char> finished;   // request already handed back
     int slot = -1;
-    hipEvent_t done = nullptr;
-    void *dev = nullptr;   // one stream-ordered allocation (inputs, workspace)
-    void *host = nullptr;  // host-mapped, coherent: pos/vel in and out, results, flags
     size_t n_part = 0;
     double *h_pos = nullptr, *h_vel = nullptr;
     hhs_system_result *h_res = nullptr;
@@ -83,7 +100,7 @@ struct hhs_server {
     std::unordered_map<long long, std::unique_ptr<hhs::Request>> all;
     std::vector<hhs::Request *> admitted_small;
     std::deque<hhs::Request *> admitted_big;
-    std::vector<hipStream_t> batch_streams;
+    std::vector<hhs::Slot> slots;
     std::vector<bool> slot_busy;
     std::vector<hhs::Batch *> inflight;
     std::vector<std::thread> workers;
@@ -115,9 +132,31 @@ void fail_request(hhs_server *s, Request *r, int status) {
 // system's final state and result there and then raises that system's flag,
 // so the scheduler can hand back every request as soon as its own system
 // is done instead of when the whole batch is.
+int grow_slot(Slot &sl, size_t dev_bytes, size_t host_bytes) {
+    if (dev_bytes > sl.dev_bytes) {
+        const size_t want = std::max(dev_bytes, 2 * sl.dev_bytes);
+        if (sl.dev) HHS_HIP(hipFreeAsync(sl.dev, sl.st));
+        sl.dev = nullptr;
+        sl.dev_bytes = 0;
+        HHS_HIP(hipMallocAsync(&sl.dev, want, sl.st));
+        sl.dev_bytes = want;
+    }
+    if (host_bytes > sl.host_bytes) {
+        const size_t want = std::max(host_bytes, 2 * sl.host_bytes);
+        if (sl.host) sl.retired_host.push_back(sl.host);
+        sl.host = sl.host_dev = nullptr;
+        sl.host_bytes = 0;
+        HHS_HIP(hipHostMalloc(&sl.host, want, hipHostMallocMapped | hipHostMallocCoherent));
+        sl.host_bytes = want;
+        HHS_HIP(hipHostGetDevicePointer(&sl.host_dev, sl.host, 0));
+    }
+    return HHS_OK;
+}
+
 int launch_batch(hhs_server *s, Batch *b) {
     const int k = static_cast<int>(b->reqs.size());
-    hipStream_t st = s->batch_streams[b->slot];
+    Slot &sl = s->slots[b->slot];
+    hipStream_t st = sl.st;
     std::vector<int> off(k + 1, 0);
     for (int q = 0; q < k; ++q) off[q + 1] = off[q] + b->reqs[q]->n;
     const size_t np = static_cast<size_t>(off[k]);
@@ -133,9 +172,8 @@ int launch_batch(hhs_server *s, Batch *b) {
     const size_t h_pos = o_ws, h_vel = h_pos + al(np * 24 + 8), h_res = h_vel + al(np * 24 + 8),
                  h_done = h_res + al(k * sizeof(hhs_system_result)),
                  h_bytes = h_done + al(k * sizeof(int));
-    HHS_HIP(hipMallocAsync(&b->dev, dev_bytes, st));
-    HHS_HIP(hipHostMalloc(&b->host, h_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    char *h = static_cast<char *>(b->host);
+    if (int rc = grow_slot(sl, dev_bytes, h_bytes)) return rc;
+    char *h = static_cast<char *>(sl.host);
     std::memcpy(h + o_off, off.data(), (k + 1) * 4);
     double *hp = reinterpret_cast<double *>(h + h_pos), *hv = reinterpret_cast<double *>(h + h_vel),
            *hm = reinterpret_cast<double *>(h + o_m), *hte = reinterpret_cast<double *>(h + o_te),
@@ -153,10 +191,8 @@ int launch_batch(hhs_server *s, Batch *b) {
     b->h_res = reinterpret_cast<hhs_system_result *>(h + h_res);
     b->h_done = reinterpret_cast<int *>(h + h_done);
     std::memset(b->h_done, 0, k * sizeof(int));
-    void *hdev = nullptr;
-    HHS_HIP(hipHostGetDevicePointer(&hdev, b->host, 0));
-    char *hd = static_cast<char *>(hdev);
-    char *d = static_cast<char *>(b->dev);
+    char *hd = static_cast<char *>(sl.host_dev);
+    char *d = static_cast<char *>(sl.dev);
     HHS_HIP(hipMemcpyAsync(d, h, o_ws, hipMemcpyHostToDevice, st));
     hhs_ensemble_desc desc{};
     desc.n_systems = k;
@@ -176,8 +212,7 @@ int launch_batch(hhs_server *s, Batch *b) {
     desc.done = reinterpret_cast<int *>(hd + h_done);
     int rc = hhs_ensemble_run_f64(&desc, off.data(), d + o_ws, wsb, st);
     if (rc) return rc;
-    HHS_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
-    HHS_HIP(hipEventRecord(b->done, st));
+    HHS_HIP(hipEventRecord(sl.done, st));
     return HHS_OK;
 }
 
@@ -207,7 +242,9 @@ bool poll_batch(hhs_server *s, Batch *b) {
     return any;
 }
 
-// Retire a batch after its launch has completed (lock held).
+// Retire a batch after its launch has completed (lock held; no HIP call).
+// A failed launch_batch may have queued work on the slot's stream before it
+// failed; the scheduler waits for the slot's stream before reusing it.
 void finish_batch(hhs_server *s, Batch *b, int rc) {
     const double t = s->now();
     for (size_t q = 0; q < b->reqs.size(); ++q) {
@@ -217,10 +254,6 @@ void finish_batch(hhs_server *s, Batch *b, int rc) {
         else
             fail_request(s, b->reqs[q], rc == HHS_OK ? HHS_ERR_HIP : rc);
     }
-    hipStream_t st = s->batch_streams[b->slot];
-    if (b->dev) (void)hipFreeAsync(b->dev, st);
-    if (b->done) (void)hipEventDestroy(b->done);
-    if (b->host) (void)hipHostFree(b->host);
     s->slot_busy[b->slot] = false;
     delete b;
 }
@@ -261,6 +294,9 @@ void scheduler_loop(hhs_server *s) {
             s->slot_busy[slot] = true;
             lk.unlock();
             const int rc = launch_batch(s, b);
+            // a launch that failed part-way leaves its queued work behind:
+            // let it drain before the slot's buffers are reused
+            if (rc) (void)hipStreamSynchronize(s->slots[slot].st);
             lk.lock();
             if (rc)
                 finish_batch(s, b, rc);
@@ -272,7 +308,7 @@ void scheduler_loop(hhs_server *s) {
         for (size_t q = 0; q < s->inflight.size();) {
             Batch *b = s->inflight[q];
             if (poll_batch(s, b)) s->cv.notify_all();
-            const hipError_t e = hipEventQuery(b->done);
+            const hipError_t e = hipEventQuery(s->slots[b->slot].done);
             if (e == hipErrorNotReady) {
                 ++q;
                 continue;
@@ -395,14 +431,17 @@ int hhs_server_create(hhs_server **out, const hhs_server_config *cfg) {
     if (s->cfg.max_level <= 0) s->cfg.max_level = 40;
     s->t0 = std::chrono::steady_clock::now();
     if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
-    s->batch_streams.resize(s->cfg.batch_streams, nullptr);
+    s->slots.resize(s->cfg.batch_streams);
     s->slot_busy.assign(s->cfg.batch_streams, false);
-    for (auto &st : s->batch_streams) {
-        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    for (auto &sl : s->slots) {
+        hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e != hipSuccess) {
             int rc = hip_status(e, "server_create: stream", __FILE__, __LINE__);
-            for (auto &x : s->batch_streams)
-                if (x) (void)hipStreamDestroy(x);
+            for (auto &x : s->slots) {
+                if (x.done) (void)hipEventDestroy(x.done);
+                if (x.st) (void)hipStreamDestroy(x.st);
+            }
             delete s;
             return rc;
         }
@@ -427,8 +466,14 @@ int hhs_server_destroy(hhs_server *s) {
     }
     s->cv.notify_all();
     for (auto &t : s->workers) t.join();
-    for (auto &st : s->batch_streams)
-        if (st) (void)hipStreamDestroy(st);
+    for (auto &sl : s->slots) {
+        if (sl.dev) (void)hipFreeAsync(sl.dev, sl.st);
+        if (sl.st) (void)hipStreamSynchronize(sl.st);
+        if (sl.host) (void)hipHostFree(sl.host);
+        for (void *h : sl.retired_host) (void)hipHostFree(h);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
     delete s;
     return HHS_OK;
 }

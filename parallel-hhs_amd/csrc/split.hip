@@ -26,7 +26,10 @@
 // The lone-fast-particle rule (src/split.cuh:321-324) needs the final count,
 // which only the last chunk knows; a follow-up kernel rewrites the slow set
 // as a plain copy when the raw fast count is 1 (it exits at once otherwise).
+#include <type_traits>
+
 #include "hhs_internal.h"
+#include "pair_math.h"
 
 namespace hhs {
 namespace {
@@ -73,12 +76,78 @@ __device__ __forceinline__ int block_sum(int x, int *s_red) {
     return t;
 }
 
+// Decision guard of the HOLD split (pair_math.h) over elements [b, e) of
+// the call's set: every timestep within the band of a pivot dt * 2^-k is
+// recomputed with the exact formula over all n particles of the set (the
+// sources of its timestep evaluation), the workgroup's threads splitting
+// the sources, and written back before any predicate is formed from it.
+// The common case (nothing flagged) costs one compare per element and one
+// barrier per tile.
+struct SplitGuard {
+    TsGuard g;
+    double eta, c_ff, eps2;  // the timestep criterion's parameters
+};
+
+__device__ void guard_chunk(const SplitGuard &gd, const Particles<double> &set, int n, int b,
+                            int e) {
+    __shared__ int s_list[kTile];
+    __shared__ int s_cnt;
+    __shared__ double s_min[kWaves];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int base = b; base < e; base += kTile) {
+        bool fl[kItems];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const int i = base + k * kBlock + t;
+            fl[k] = i < e && guard_near_pivot(gd.g, set.timestep[i]);
+            any = any || fl[k];
+        }
+        if (!__syncthreads_or(any)) continue;
+        if (t == 0) s_cnt = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (fl[k]) s_list[atomicAdd(&s_cnt, 1)] = base + k * kBlock + t;
+        __syncthreads();
+        const int cnt = s_cnt;
+        for (int q = 0; q < cnt; ++q) {
+            const int s = s_list[q];
+            const double xi = set.pos[3 * s], yi = set.pos[3 * s + 1], zi = set.pos[3 * s + 2];
+            const double vxi = set.vel[3 * s], vyi = set.vel[3 * s + 1], vzi = set.vel[3 * s + 2];
+            const double mi = set.m[s];
+            double best = TsConst<double>::huge;
+            for (int j = t; j < n; j += kBlock)
+                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, set.pos[3 * j],
+                                      set.pos[3 * j + 1], set.pos[3 * j + 2], set.vel[3 * j],
+                                      set.vel[3 * j + 1], set.vel[3 * j + 2], set.m[j], gd.eta,
+                                      gd.c_ff, gd.eps2, j != s, best);
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const double o = __shfl_xor(best, d, 64);
+                best = o < best ? o : best;
+            }
+            if (lane == 0) s_min[wave] = best;
+            __syncthreads();
+            if (t == 0) {
+                double m = s_min[0];
+#pragma unroll
+                for (int w = 1; w < kWaves; ++w) m = s_min[w] < m ? s_min[w] : m;
+                set.timestep[s] = m;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <class Op>
 __global__ __launch_bounds__(kBlock) void count_kernel(Op op, int n, int per_chunk,
                                                        int *counts) {
     __shared__ int s_red[kWaves];
     const int b = blockIdx.x * per_chunk;
     const int e_end = min(b + per_chunk, n);
+    if constexpr (Op::kGuard)
+        if (op.guard) guard_chunk(op.gd, op.total, n, b, e_end);
     int c = 0;
 #pragma unroll 8
     for (int e = b + static_cast<int>(threadIdx.x); e < e_end; e += kBlock) c += op.value_at(e);
@@ -99,6 +168,9 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
     const int lane = t & 63;
     const int wave = t >> 6;
     const int g = blockIdx.x;
+    // a single chunk has no count pass: the guard runs here
+    if constexpr (Op::kGuard)
+        if (op.guard && gridDim.x == 1) guard_chunk(op.gd, op.total, n, 0, n);
 
     int offset = 0;
     for (int j = t; j < g; j += kBlock) offset += ws.counts[j];
@@ -155,6 +227,7 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
 // ------------------------------------------------------------------ ops
 
 struct ScanI32Op {
+    static constexpr bool kGuard = false;
     const int *__restrict__ in;
     int *__restrict__ out;
     using Item = int;
@@ -172,6 +245,9 @@ template <typename T> struct SplitRec {
 };
 
 template <typename T> struct SplitOp {
+    // the decision guard exists for FP64 sets (HOLD's approximate timestep
+    // modes)
+    static constexpr bool kGuard = std::is_same<T, double>::value;
     Particles<T> total, slow, fast;
     int *predicate, *scanned;
     T pivot;
@@ -182,6 +258,9 @@ template <typename T> struct SplitOp {
     int drift;
     T h;
     int *raw;
+    // HOLD only: the decision guard, when guard != 0
+    int guard;
+    SplitGuard gd;
     using Item = SplitRec<T>;
     // src/split.cuh:115 -- NaN compares false and counts as slow.
     __device__ int value_at(int e) const { return total.timestep[e] <= pivot ? 1 : 0; }
@@ -227,6 +306,7 @@ template <typename T> struct SplitOp {
 };
 
 template <typename T> struct SplitValuesOp {
+    static constexpr bool kGuard = false;
     const T *__restrict__ values;
     T *slow, *fast;
     T pivot;
@@ -371,12 +451,26 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
 // `part` and the raw fast count to `raw` (host-mapped memory, read after a
 // stream sync), and no lone-particle follow-up is launched -- the caller
 // runs launch_lone_fast_fixup when *raw == 1.
+//
+// guard_band > 0 (FP64 sets; calls that have just computed their timesteps
+// in an approximate mode): the decision guard with that band, the pivot as
+// the call's dt and the criterion's eta and eps2, runs in the count pass.
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
-                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st) {
+                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st, double guard_band,
+                      double eta, double eps2) {
     HHS_REQUIRE(n > 0 && part && raw, "split_hold: bad arguments");
     SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, raw};
+    if constexpr (SplitOp<T>::kGuard) {
+        if (guard_band > 0.0) {
+            op.guard = 1;
+            op.gd.g = make_ts_guard(static_cast<double>(pivot), guard_band);
+            op.gd.eta = eta;
+            op.gd.c_ff = eta / TsConst<double>::sqrt2;
+            op.gd.eps2 = eps2;
+        }
+    }
     return launch_scan_op<true>(op, n, ws, ws_bytes, st);
 }
 
@@ -392,7 +486,8 @@ int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, 
 
 template int launch_split_hold<double>(const Particles<double> &, const Particles<double> &,
                                        const Particles<double> &, double, int, hhs_partition *,
-                                       int *, double, void *, size_t, hipStream_t);
+                                       int *, double, void *, size_t, hipStream_t, double,
+                                       double, double);
 template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
                                             int, double, hipStream_t);
 

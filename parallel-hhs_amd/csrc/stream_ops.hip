@@ -5,6 +5,7 @@
 // counterpart.  Drift and update are HBM-bound (72 B and 132 B per
 // particle); they run without FMA contraction so FP64 results are bit-exact
 // against the same expression on the CPU.
+#include <cmath>
 #include <vector>
 
 #include "hhs_internal.h"
@@ -102,7 +103,10 @@ __global__ __launch_bounds__(256) void energy_rows_kernel(const double *__restri
         if (j > i && r2 != 0.0) s += t;
     }
     if (i < n) {
-        row[i] = m[i] * s;
+        // an infinite coordinate would only zero its pair terms: mark the
+        // row so that the sum reports the state as non-finite
+        const bool fin = isfinite(xi) && isfinite(yi) && isfinite(zi);
+        row[i] = fin ? m[i] * s : __builtin_nan("");
         const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
         kin[i] = 0.5 * m[i] * (vx * vx + vy * vy + vz * vz);
     }
@@ -242,6 +246,14 @@ int hhs_energy_f64(const double *pos, const double *vel, const double *m, int n,
     if (rc) return rc;
     *kinetic = host[0];
     *potential = host[1];
+    // a NaN or Inf anywhere in the state (positions through the row marks,
+    // velocities and masses through the sums) or an overflow reaches one of
+    // the two sums
+    if (!std::isfinite(host[0]) || !std::isfinite(host[1])) {
+        set_error("energy: non-finite result (kinetic %g, potential %g) for n=%d", host[0],
+                  host[1], n);
+        return HHS_ERR_NONFINITE;
+    }
     return HHS_OK;
 }
 

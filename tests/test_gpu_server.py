@@ -82,3 +82,50 @@ def test_server_refuses_oversized_request(hhs, torch_cuda):
         with pytest.raises(hhs.HHSError) as ei:
             srv.submit(p, v, m, 0.1, 0.1)
     assert ei.value.code == hhs.HHS_ERR_BUDGET
+
+
+def test_server_full_size_range_matches_direct_hold(hhs, oracle, torch_cuda):
+    """The served workload's size range, N in [64, 65536] with one request
+    above 32768, FP32 timesteps: every served result equals bit for bit the
+    same system integrated on its own (a one-system ensemble up to
+    small_max_n = 512, hhs_hold_f64 above), and a small and a mid-size
+    request track the FP64 CPU HOLD oracle (same calls, positions within
+    1e-6 of the rms)."""
+    sizes = [64, 300, 700, 3000, 12000, 40000]
+    t_ends = [0.5, 0.25, 0.125, 0.1, 0.1, 0.1]
+    dtr, mode = 1.0 / 64, hhs.TIMESTEP_FP32
+    reqs = [plummer(hhs, n, seed=5500 + k, eps2=EPS * EPS) for k, n in enumerate(sizes)]
+    with hhs.Server(ETA, EPS, max_n=65536, max_level=40, fast_timesteps=mode) as srv:
+        tickets = [srv.submit(p, v, m, te, dtr) for (p, v, m), te in zip(reqs, t_ends)]
+        served = []
+        for t in tickets:
+            info = srv.wait(t)
+            assert info["state"] == 3, info
+            served.append((srv.fetch(t), info))
+    for k, ((p, v, m), te) in enumerate(zip(reqs, t_ends)):
+        (ps, vs), info = served[k]
+        assert info["path"] == (0 if sizes[k] <= 512 else 1)
+        if sizes[k] <= 512:
+            ens = hhs.Ensemble([(p, v, m)], te, dtr, ETA, EPS, max_level=40, fast_timesteps=mode)
+            ens.run()
+            torch_cuda.cuda.synchronize()
+            pd, vd = ens.state(0)
+            calls = int(ens.results()[0]["calls"])
+        else:
+            d = hhs.DeviceParticles.from_host(p, v, m)
+            g = hhs.Globals(len(m), ETA, EPS, max_level=40, fast_timesteps=mode)
+            g.hold(d, te, dtr)
+            torch_cuda.cuda.synchronize()
+            pd, vd = d.pos.cpu().numpy(), d.vel.cpu().numpy()
+            calls = g.stats()["calls"]
+        assert pd.tobytes() == ps.tobytes() and vd.tobytes() == vs.tobytes(), sizes[k]
+        assert calls == info["result"]["calls"]
+    for k in (0, 2):
+        (p, v, m), te = reqs[k], t_ends[k]
+        host = oracle.HostParticles(len(m), p, v, m)
+        rc, ost = oracle.hold(host, te, dtr, ETA, EPS * EPS)
+        assert rc == 0
+        (ps, vs), info = served[k]
+        assert info["result"]["calls"] == ost.calls
+        assert np.abs(ps - host.pos).max() <= 1e-6 * np.sqrt((host.pos**2).mean())
+        assert np.abs(vs - host.vel).max() <= 1e-6 * np.sqrt((host.vel**2).mean())

@@ -206,3 +206,20 @@ def test_timestep_autotune_installs_a_valid_config(hhs, oracle, torch_cuda, tmp_
             assert np.all(np.abs(got - ref) <= 1e-6 * ref)
     finally:
         hhs.autotune_clear()
+
+
+def test_energy_reports_nonfinite_state(hhs, torch_cuda):
+    """A NaN in the state makes the FP64 energy diagnostic return
+    HHS_ERR_NONFINITE instead of a NaN energy."""
+    pos, vel, m = hhs.ic_plummer(300, 17, 0, True, 0.0)
+    vel[123, 1] = np.nan
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    with pytest.raises(hhs.HHSError) as ei:
+        hhs.energy(d.pos, d.vel, d.m, 1e-8)
+    assert ei.value.code == hhs.HHS_ERR_NONFINITE
+    pos[7, 0] = np.inf
+    vel[123, 1] = 0.0
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    with pytest.raises(hhs.HHSError) as ei:
+        hhs.energy(d.pos, d.vel, d.m, 1e-8)
+    assert ei.value.code == hhs.HHS_ERR_NONFINITE
