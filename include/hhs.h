@@ -334,9 +334,20 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
                       double stime, double etime, double dt, int calc_timestep,
                       void *stream);
 
+/* hold_step<float,BLOCKSIZE> (src/hold.cuh:51-53 with T = float): the same
+ * recursion on an FP32 particle set.  Timesteps in FP32 with the formula's
+ * IEEE operations (the globals' timestep mode applies to FP64 sets only),
+ * MIXED32 kicks on FP32 separations, FP32 drifts; host recursion only (no
+ * device subtrees).  The globals' per-level buffers hold either precision,
+ * one hold_step call at a time. */
+int hhs_hold_step_f32(hhs_globals *g, hhs_particles_f32 *total, int n, int clevel, float stime,
+                      float etime, float dt, int calc_timestep, void *stream);
+
 /* hold<T> (declared, never defined: src/hold.cuh:20-35): root steps of
  * dt_root (timesteps recomputed at each root step) until t_end. */
 int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,
+                 void *stream);
+int hhs_hold_f32(hhs_globals *g, hhs_particles_f32 *sys, float t_end, float dt_root,
                  void *stream);
 
 /* ------------------------------------------- driver and snapshots -- */

@@ -144,28 +144,48 @@ void update_system(particle_system<T> *prev_level_sys, particle_system<T> *cur_l
 }
 
 // ------------------------------------------------------------------ HOLD
-// hold_step (src/hold.cuh:51-181).  FP64 systems.  Per-level buffers are
-// owned by `globals` and reused; globals->simtime tracks the reference's
-// host-side simtime (src/hold.cuh:106-114).
+// hold_step (src/hold.cuh:51-181), FP64 and FP32 particle systems.
+// Per-level buffers are owned by `globals` and reused; globals->simtime
+// tracks the reference's host-side simtime (src/hold.cuh:106-114).
 template <typename T, int BLOCKSIZE>
 void hold_step(int clevel, nbodysystem_globals<T> *globals, particle_system<T> *total,
                const int N_total, T stime, T etime, T dt, bool calc_timestep) {
-    static_assert(std::is_same<T, double>::value, "hold_step: FP64 particle systems only");
+    static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                  "hold_step: FP64 or FP32 particle systems");
     auto v = total->c_view();
-    hhs_globals_set_simtime(globals->handle, globals->simtime);
-    gpuErrchk(hhs_hold_step_f64(globals->handle, &v, N_total, clevel, stime, etime, dt,
-                                calc_timestep ? 1 : 0, nullptr));
-    globals->simtime = hhs_globals_simtime(globals->handle);
+    hhs_globals_set_simtime(globals->handle, static_cast<double>(globals->simtime));
+    if constexpr (std::is_same<T, double>::value)
+        gpuErrchk(hhs_hold_step_f64(globals->handle, &v, N_total, clevel, stime, etime, dt,
+                                    calc_timestep ? 1 : 0, nullptr));
+    else
+        gpuErrchk(hhs_hold_step_f32(globals->handle, &v, N_total, clevel, stime, etime, dt,
+                                    calc_timestep ? 1 : 0, nullptr));
+    globals->simtime = static_cast<T>(hhs_globals_simtime(globals->handle));
 }
 
-// hold (declared, never defined in the reference: src/hold.cuh:20-35).  The
-// declared signature has no access to eta/eps, so this definition takes the
-// globals object: root steps of dt from stime to etime, with the timesteps
-// recomputed at every root step.
+// hold, with the signature the reference declares and never defines
+// (src/hold.cuh:20-23): root steps of dt from stime while t < etime, each a
+// hold_step at level clevel.  calc_timestep applies to the first root step
+// (false: sys->timestep already holds the timesteps of the current state);
+// every later root step recomputes them, since the positions have moved.
+// eta and eps come from buffers->params; the HOLD state lives in
+// buffers->globals (created on the first call).
+template <typename T>
+void hold(int clevel, particle_system<T> *sys, nbodysystem_buffers<T> *buffers,
+          const int N_total, T stime, T etime, T dt, bool calc_timestep) {
+    nbodysystem_globals<T> *g = buffers->hold_globals();
+    bool calc = calc_timestep;
+    for (T t = stime; t < etime; t += dt) {
+        hold_step<T, 0>(clevel, g, sys, N_total, t, t + dt, dt, calc);
+        calc = true;
+    }
+}
+
+// hold over an explicit globals object (eta and eps from globals->params):
+// the same root-step loop, timesteps computed at every root step.
 template <typename T>
 void hold(int clevel, nbodysystem_globals<T> *globals, particle_system<T> *sys,
           const int N_total, T stime, T etime, T dt) {
-    static_assert(std::is_same<T, double>::value, "hold: FP64 particle systems only");
     for (T t = stime; t < etime; t += dt)
         hold_step<T, 0>(clevel, globals, sys, N_total, t, t + dt, dt, true);
 }

@@ -245,27 +245,44 @@ template <typename T> class nbodysystem_globals {
 // src/particle_system.cuh:107-132.  The reference's destructor frees `part`
 // and the arrays but leaks the two particle_system objects; this one frees
 // them all.
+//
+// The reference declares hold<T>(clevel, sys, buffers, ...) with the
+// buffers as its only state (src/hold.cuh:20-23), so the buffers here also
+// carry the integrator parameters (`params`: eta, eps; by default 0.01, the
+// value of every benchmark configuration, and 0, the reference kick's
+// softening) and, from the first hold() call on, the HOLD state of the C ABI
+// (`globals`, whose simtime tracks the integration).
 template <typename T> class nbodysystem_buffers {
   public:
     particle_system<T> *slow;
     particle_system<T> *fast;
     int N_total;
     partition *part;
+    nbody_params<T> *params;
+    nbodysystem_globals<T> *globals = nullptr;
 
-    explicit nbodysystem_buffers(int N_init) : N_total(N_init) {
+    explicit nbodysystem_buffers(int N_init, T eta = T(0.01), T eps = T(0)) : N_total(N_init) {
         slow = new particle_system<T>(N_init, 'g');
         fast = new particle_system<T>(N_init, 'g');
         slow->gpu_alloc();
         fast->gpu_alloc();
         gpuErrchk(hhs_host_alloc_mapped(reinterpret_cast<void **>(&part), sizeof(partition)));
         part->N_slow = part->N_fast = 0;
+        params = new nbody_params<T>{eta, eps};
     }
     ~nbodysystem_buffers() {
+        delete globals;
+        delete params;
         hhs_host_free_mapped(part);
         slow->gpu_free();
         fast->gpu_free();
         delete slow;
         delete fast;
+    }
+    // The HOLD state for hold(): created with params on first use.
+    nbodysystem_globals<T> *hold_globals() {
+        if (!globals) globals = new nbodysystem_globals<T>(N_total, params->eta, params->eps);
+        return globals;
     }
     nbodysystem_buffers(const nbodysystem_buffers &) = delete;
     nbodysystem_buffers &operator=(const nbodysystem_buffers &) = delete;

@@ -25,6 +25,7 @@
 //    bound when timesteps collapse (two coincident particles with eps = 0).
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "hhs_internal.h"
@@ -32,10 +33,20 @@
 
 namespace hhs {
 
+// One allocation per level holding its slow and fast sets, carved for FP64
+// records and, over the same memory, for FP32 records (half the size); a
+// hold_step call uses one precision throughout.
 struct LevelBuffers {
-    void *block = nullptr;  // one allocation holding both sets
+    void *block = nullptr;
     Particles<double> slow{}, fast{};
+    Particles<float> slow32{}, fast32{};
+    template <typename T> const Particles<T> &slow_set() const;
+    template <typename T> const Particles<T> &fast_set() const;
 };
+template <> inline const Particles<double> &LevelBuffers::slow_set<double>() const { return slow; }
+template <> inline const Particles<double> &LevelBuffers::fast_set<double>() const { return fast; }
+template <> inline const Particles<float> &LevelBuffers::slow_set<float>() const { return slow32; }
+template <> inline const Particles<float> &LevelBuffers::fast_set<float>() const { return fast32; }
 
 }  // namespace hhs
 
@@ -74,19 +85,20 @@ struct hhs_globals {
 namespace hhs {
 namespace {
 
-// Carves one particle set of capacity n out of `p` (96 B per particle).
-char *carve(char *p, int n, Particles<double> &out) {
+// Carves one particle set of capacity n out of `p` (96 B per particle in
+// FP64, 56 B in FP32).
+template <typename T> char *carve(char *p, int n, Particles<T> &out) {
     auto take = [&](size_t bytes) {
         char *r = p;
         p += (bytes + 255) & ~static_cast<size_t>(255);
         return r;
     };
     const size_t nn = static_cast<size_t>(n);
-    out.pos = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
-    out.vel = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
-    out.acc = reinterpret_cast<double *>(take(3 * nn * sizeof(double)));
-    out.timestep = reinterpret_cast<double *>(take(nn * sizeof(double)));
-    out.m = reinterpret_cast<double *>(take(nn * sizeof(double)));
+    out.pos = reinterpret_cast<T *>(take(3 * nn * sizeof(T)));
+    out.vel = reinterpret_cast<T *>(take(3 * nn * sizeof(T)));
+    out.acc = reinterpret_cast<T *>(take(3 * nn * sizeof(T)));
+    out.timestep = reinterpret_cast<T *>(take(nn * sizeof(T)));
+    out.m = reinterpret_cast<T *>(take(nn * sizeof(T)));
     out.id = reinterpret_cast<int *>(take(nn * sizeof(int)));
     out.parent_id = reinterpret_cast<int *>(take(nn * sizeof(int)));
     out.n = n;
@@ -113,6 +125,9 @@ int ensure_level(hhs_globals *g, int level) {
     char *p = static_cast<char *>(lb.block);
     p = carve(p, g->n_total, lb.slow);
     carve(p, g->n_total, lb.fast);
+    p = static_cast<char *>(lb.block);
+    p = carve(p, g->n_total, lb.slow32);
+    carve(p, g->n_total, lb.fast32);
     return HHS_OK;
 }
 
@@ -205,15 +220,34 @@ int alloc_subtree(hhs_globals *g, int n_max) {
     return HHS_OK;
 }
 
-int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
-             bool calc, hipStream_t st) {
+// Timesteps of the call's set: FP64 sets in the globals' mode (symmetric
+// kernel from kSymMinN particles when its workspace exists); FP32 sets in
+// FP32 with the formula's IEEE operations (the reference with T = float).
+int call_timesteps(hhs_globals *g, const Particles<double> &total, int n, hipStream_t st) {
+    if (g->ts_ws && n >= kSymMinN)
+        return launch_timesteps_sym(g->timestep_mode, total.pos, total.vel, total.m,
+                                    total.timestep, n, g->eta, g->eps2, g->ts_ws, g->ts_ws_bytes,
+                                    st);
+    return launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m, total.timestep,
+                                 n, g->eta, g->eps2, st);
+}
+int call_timesteps(hhs_globals *g, const Particles<float> &total, int n, hipStream_t st) {
+    return launch_timesteps<float>(total.pos, total.vel, total.m, total.timestep, n,
+                                   static_cast<float>(g->eta), static_cast<float>(g->eps2), st);
+}
+
+template <typename T>
+int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt, bool calc,
+             hipStream_t st) {
+    constexpr bool f64 = std::is_same<T, double>::value;
     if (clevel > g->max_level) {
         set_error("hold_step: recursion level %d exceeds max_level %d (timesteps collapsed?)",
                   clevel, g->max_level);
         return HHS_ERR_DEPTH;
     }
     hhs_hold_stats &S = g->stats;
-    if (use_subtree(g, n, clevel)) return run_subtree(g, total, n, clevel, dt, calc, st);
+    if constexpr (f64)
+        if (use_subtree(g, n, clevel)) return run_subtree(g, total, n, clevel, dt, calc, st);
     S.calls++;
     if (clevel < 64) S.calls_per_level[clevel]++;
     if (clevel > S.max_level) S.max_level = clevel;
@@ -221,77 +255,86 @@ int hold_rec(hhs_globals *g, const Particles<double> &total, int n, int clevel, 
     int rc;
     // src/hold.cuh:74-76
     if (calc) {
-        if (g->ts_ws && n >= kSymMinN)
-            rc = launch_timesteps_sym(g->timestep_mode, total.pos, total.vel, total.m,
-                                      total.timestep, n, g->eta, g->eps2, g->ts_ws,
-                                      g->ts_ws_bytes, st);
-        else
-            rc = launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m,
-                                       total.timestep, n, g->eta, g->eps2, st);
-        if (rc) return rc;
+        if ((rc = call_timesteps(g, total, n, st))) return rc;
         S.ts_pairs += static_cast<long long>(n) * (n - 1);
     }
     if ((rc = ensure_level(g, clevel))) return rc;
-    const LevelBuffers lb = g->levels[clevel];  // by value: plain pointers
-    const double h = dt * 0.5;  // the half-drift of src/hold.cuh:123-124,158-159
+    const Particles<T> slow = g->levels[clevel].slow_set<T>();  // by value: plain pointers
+    const Particles<T> fast = g->levels[clevel].fast_set<T>();
+    const T h = dt * T(0.5);  // the half-drift of src/hold.cuh:123-124,158-159
     // src/hold.cuh:79-83 with the first slow half-drift, then the count
-    // read-back (src/split.cuh:430)
-    // with the decision guard after a timestep evaluation in an approximate
-    // mode, so that the slow/fast decisions are the exact mode's
-    rc = launch_split_hold<double>(total, lb.slow, lb.fast, dt, n, g->part_map + clevel,
-                                   g->raw_map + clevel, h, g->scan_ws, g->scan_ws_bytes, st,
-                                   calc ? guard_band(g) : 0.0, g->eta, g->eps2);
+    // read-back (src/split.cuh:430); FP64 sets run the decision guard after
+    // a timestep evaluation in an approximate mode, so that the slow/fast
+    // decisions are the exact mode's
+    const double band = (f64 && calc) ? guard_band(g) : 0.0;
+    rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel, g->raw_map + clevel,
+                              h, g->scan_ws, g->scan_ws_bytes, st, band, g->eta, g->eps2);
     if (rc) return rc;
     HHS_HIP(hipStreamSynchronize(st));
     const int ns = g->part_host[clevel].N_slow;
     const int nf = g->part_host[clevel].N_fast;
     // a lone fast particle stays slow (src/split.cuh:321-324): the slow set
     // becomes the whole set, drifted
-    if (g->raw_host[clevel] == 1 && (rc = launch_lone_fast_fixup<double>(total, lb.slow, n, h, st)))
+    if (g->raw_host[clevel] == 1 && (rc = launch_lone_fast_fixup<T>(total, slow, n, h, st)))
         return rc;
 
     // src/hold.cuh:106-114
-    if (nf == 0) g->simtime += dt;
+    if (nf == 0) g->simtime += static_cast<double>(dt);
     // src/hold.cuh:116-121
-    if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, false, st))) return rc;
+    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), false, st))) return rc;
     if (ns > 0) {
         // src/hold.cuh:123-124: done by the split
         // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow.
         // In the exact FP64 mode the S<-S and S<-F kicks are two velocity
         // updates, as in the reference, so the sums match the CPU order.
         // Otherwise one launch kicks both: the fast sinks ride along as the
-        // kick's second sink group, fed by the slow segment only.
-        const int prec = g->active_kick;
+        // kick's second sink group, fed by the slow segment only.  FP32 sets
+        // always use the MIXED32 arithmetic.
+        const int prec = f64 ? g->active_kick : HHS_KICK_MIXED32;
         const bool exact = prec == HHS_KICK_F64_EXACT;
-        KickArgs<double> ks{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,
-                            lb.slow.pos, lb.slow.m,  ns,          lb.fast.pos,
-                            lb.fast.m,   exact ? 0 : nf, dt,      g->eps2,
-                            prec};
+        KickArgs<T> ks{slow.pos, slow.vel, slow.acc, ns,  slow.pos, slow.m, ns,      fast.pos,
+                       fast.m,   exact ? 0 : nf,      dt, g->eps2,  prec};
         if (!exact && nf > 0) {
-            ks.sb_pos = lb.fast.pos;
-            ks.sb_vel = lb.fast.vel;
-            ks.sb_acc = lb.fast.acc;
+            ks.sb_pos = fast.pos;
+            ks.sb_vel = fast.vel;
+            ks.sb_acc = fast.acc;
             ks.n_sink_b = nf;
         }
-        if ((rc = launch_kick<double>(ks, st))) return rc;
+        if ((rc = launch_kick<T>(ks, st))) return rc;
         if (exact && nf > 0) {
-            KickArgs<double> ksf{lb.slow.pos, lb.slow.vel, lb.slow.acc, ns,      lb.fast.pos,
-                                 lb.fast.m,   nf,          nullptr,     nullptr, 0,
-                                 dt,          g->eps2,     prec};
-            if ((rc = launch_kick<double>(ksf, st))) return rc;
-            KickArgs<double> kf{lb.fast.pos, lb.fast.vel, lb.fast.acc, nf,      lb.slow.pos,
-                                lb.slow.m,   ns,          nullptr,     nullptr, 0,
-                                dt,          g->eps2,     prec};
-            if ((rc = launch_kick<double>(kf, st))) return rc;
+            KickArgs<T> ksf{slow.pos, slow.vel, slow.acc, ns, fast.pos, fast.m, nf, nullptr,
+                            nullptr,  0,        dt,       g->eps2,  prec};
+            if ((rc = launch_kick<T>(ksf, st))) return rc;
+            KickArgs<T> kf{fast.pos, fast.vel, fast.acc, nf, slow.pos, slow.m, ns, nullptr,
+                           nullptr,  0,        dt,       g->eps2,  prec};
+            if ((rc = launch_kick<T>(kf, st))) return rc;
         }
         S.interactions += static_cast<long long>(ns) * (ns + nf) + static_cast<long long>(nf) * ns;
         // src/hold.cuh:158-159: done by the scatter-back below
     }
     // src/hold.cuh:168-173
-    if (nf > 0 && (rc = hold_rec(g, lb.fast, nf, clevel + 1, dt / 2, true, st))) return rc;
-    // src/hold.cuh:177-178 (the deeper levels reuse lb, so scatter back now),
-    // with the second slow half-drift
-    return launch_update_pair<double>(total, lb.fast, nf, lb.slow, ns, h, st);
+    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), true, st))) return rc;
+    // src/hold.cuh:177-178 (the deeper levels reuse the level's sets, so
+    // scatter back now), with the second slow half-drift
+    return launch_update_pair<T>(total, fast, nf, slow, ns, h, st);
+}
+
+// hold_step on one particle set of either precision (the C entry points).
+template <typename T, typename P>
+int hold_step_c(hhs_globals *g, P *total, int n, int clevel, T dt, int calc_timestep,
+                void *stream) {
+    HHS_REQUIRE(g && total, "hold_step: null argument");
+    HHS_REQUIRE(n > 0 && n <= total->n && n <= g->n_total,
+                "hold_step: n=%d outside [1, min(%d, %d)]", n, total->n, g->n_total);
+    HHS_REQUIRE(clevel >= 0, "hold_step: negative level");
+    HHS_REQUIRE(dt > T(0), "hold_step: dt must be positive");
+    HHS_REQUIRE(total->pos && total->vel && total->m && total->timestep,
+                "hold_step: null field in total");
+    hipStream_t st = as_stream(stream);
+    g->active_kick = resolve_kick(g);
+    const int rc = hold_rec<T>(g, from_c(*total), n, clevel, dt, calc_timestep != 0, st);
+    const int rs = fetch_subtree_stats(g, st);
+    return rc != HHS_OK ? rc : rs;
 }
 
 }  // namespace
@@ -434,18 +477,27 @@ int hhs_hold_step_f64(hhs_globals *g, hhs_particles_f64 *total, int n, int cleve
                       double stime, double etime, double dt, int calc_timestep, void *stream) {
     (void)stime;
     (void)etime;  // the system is autonomous; the reference never reads them
-    HHS_REQUIRE(g && total, "hold_step: null argument");
-    HHS_REQUIRE(n > 0 && n <= total->n && n <= g->n_total,
-                "hold_step: n=%d outside [1, min(%d, %d)]", n, total->n, g->n_total);
-    HHS_REQUIRE(clevel >= 0, "hold_step: negative level");
-    HHS_REQUIRE(dt > 0.0, "hold_step: dt must be positive");
-    HHS_REQUIRE(total->pos && total->vel && total->m && total->timestep,
-                "hold_step: null field in total");
-    hipStream_t st = as_stream(stream);
-    g->active_kick = resolve_kick(g);
-    const int rc = hold_rec(g, from_c(*total), n, clevel, dt, calc_timestep != 0, st);
-    const int rs = fetch_subtree_stats(g, st);
-    return rc != HHS_OK ? rc : rs;
+    return hold_step_c<double>(g, total, n, clevel, dt, calc_timestep, stream);
+}
+
+int hhs_hold_step_f32(hhs_globals *g, hhs_particles_f32 *total, int n, int clevel, float stime,
+                      float etime, float dt, int calc_timestep, void *stream) {
+    (void)stime;
+    (void)etime;
+    return hold_step_c<float>(g, total, n, clevel, dt, calc_timestep, stream);
+}
+
+int hhs_hold_f32(hhs_globals *g, hhs_particles_f32 *sys, float t_end, float dt_root,
+                 void *stream) {
+    HHS_REQUIRE(g && sys, "hold: null argument");
+    HHS_REQUIRE(dt_root > 0.0f, "hold: dt_root must be positive");
+    float t = 0.0f;
+    while (t < t_end) {
+        int rc = hhs_hold_step_f32(g, sys, sys->n, 0, t, t + dt_root, dt_root, 1, stream);
+        if (rc) return rc;
+        t += dt_root;
+    }
+    return HHS_OK;
 }
 
 int hhs_hold_f64(hhs_globals *g, hhs_particles_f64 *sys, double t_end, double dt_root,

@@ -488,8 +488,14 @@ template int launch_split_hold<double>(const Particles<double> &, const Particle
                                        const Particles<double> &, double, int, hhs_partition *,
                                        int *, double, void *, size_t, hipStream_t, double,
                                        double, double);
+template int launch_split_hold<float>(const Particles<float> &, const Particles<float> &,
+                                      const Particles<float> &, float, int, hhs_partition *,
+                                      int *, float, void *, size_t, hipStream_t, double, double,
+                                      double);
 template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
                                             int, double, hipStream_t);
+template int launch_lone_fast_fixup<float>(const Particles<float> &, const Particles<float> &,
+                                           int, float, hipStream_t);
 
 template int launch_split<double>(const Particles<double> &, const Particles<double> &,
                                   const Particles<double> &, int *, int *, double, int,

@@ -188,6 +188,8 @@ int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int n
 template int launch_update_pair<double>(const Particles<double> &, const Particles<double> &,
                                         int, const Particles<double> &, int, double,
                                         hipStream_t);
+template int launch_update_pair<float>(const Particles<float> &, const Particles<float> &, int,
+                                       const Particles<float> &, int, float, hipStream_t);
 
 }  // namespace hhs
 

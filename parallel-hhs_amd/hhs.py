@@ -194,6 +194,8 @@ _SIGS = {
     "hhs_globals_reset_stats": (_i, [_vp]),
     "hhs_hold_step_f64": (_i, [_vp, _P64, _i, _i, _d, _d, _d, _i, _vp]),
     "hhs_hold_f64": (_i, [_vp, _P64, _d, _d, _vp]),
+    "hhs_hold_step_f32": (_i, [_vp, _P32, _i, _i, _f, _f, _f, _i, _vp]),
+    "hhs_hold_f32": (_i, [_vp, _P32, _f, _f, _vp]),
     "hhs_ic_plummer": (_i, [_i, C.c_uint64, _i, _i, _d, _vp, _vp, _vp]),
     "hhs_ensemble_workspace_bytes": (_sz, [_i, C.c_longlong]),
     "hhs_ensemble_run_f64": (_i, [C.POINTER(EnsembleDesc), _vp, _vp, _sz, _vp]),
@@ -510,14 +512,17 @@ class Globals:
 
     def hold_step(self, sys: DeviceParticles, dt: float, calc_timestep=True, clevel=0, n=None,
                   stream=None):
+        """hhs_hold_step_f64 / _f32 by the particle set's precision."""
         n = sys.n if n is None else n
         s = sys.struct()
-        check(_lib.hhs_hold_step_f64(self.h, C.byref(s), n, clevel, 0.0, dt, dt,
-                                     1 if calc_timestep else 0, _stream(stream)))
+        check(getattr(_lib, "hhs_hold_step_" + _sfx(sys))(self.h, C.byref(s), n, clevel, 0.0, dt,
+                                                        dt, 1 if calc_timestep else 0,
+                                                        _stream(stream)))
 
     def hold(self, sys: DeviceParticles, t_end: float, dt_root: float, stream=None):
         s = sys.struct()
-        check(_lib.hhs_hold_f64(self.h, C.byref(s), t_end, dt_root, _stream(stream)))
+        check(getattr(_lib, "hhs_hold_" + _sfx(sys))(self.h, C.byref(s), t_end, dt_root,
+                                                   _stream(stream)))
 
     @property
     def root_steps(self) -> int:

@@ -196,6 +196,139 @@ void test_split_update_roundtrip() {
     back.gpu_free();
 }
 
+// Host FP64 energy of a downloaded system (any T), softened with eps2.
+template <typename T> double host_energy(const particle_system<T> &h, int N, double eps2) {
+    double k = 0.0, u = 0.0;
+    for (int i = 0; i < N; i++) {
+        const double vx = h.vel[i].x, vy = h.vel[i].y, vz = h.vel[i].z;
+        k += 0.5 * double(h.m[i]) * (vx * vx + vy * vy + vz * vz);
+        for (int j = i + 1; j < N; j++) {
+            const double dx = double(h.pos[i].x) - h.pos[j].x, dy = double(h.pos[i].y) - h.pos[j].y,
+                         dz = double(h.pos[i].z) - h.pos[j].z;
+            u -= double(h.m[i]) * h.m[j] / std::sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        }
+    }
+    return k + u;
+}
+
+template <typename T>
+void load_plummer(particle_system<T> &h, int N, uint64_t seed, double eps2) {
+    std::vector<double> p(3 * N), v(3 * N), m(N);
+    gpuErrchk(hhs_ic_plummer(N, seed, 0, 1, eps2, p.data(), v.data(), m.data()));
+    for (int i = 0; i < N; i++) {
+        h.pos[i] = vec3<T>(T(p[3 * i]), T(p[3 * i + 1]), T(p[3 * i + 2]));
+        h.vel[i] = vec3<T>(T(v[3 * i]), T(v[3 * i + 1]), T(v[3 * i + 2]));
+        h.acc[i] = vec3<T>();
+        h.m[i] = T(m[i]);
+        h.timestep[i] = T(0);
+        h.id[i] = i;
+    }
+}
+
+// hold_step<float, BS> (src/hold.cuh:51-53 with T = float): the FP32
+// integration conserves energy to the integrator's error, advances simtime
+// by dt per root step, and stays within 1e-5 of the rms of the FP64 run.
+void test_hold_step_f32() {
+    const int N = 256;
+    const double eps = 0.01;
+    particle_system<float> h(N, 'c'), d(N, 'g'), out(N, 'c');
+    particle_system<double> h64(N, 'c'), d64(N, 'g'), out64(N, 'c');
+    h.cpu_alloc();
+    out.cpu_alloc();
+    h64.cpu_alloc();
+    out64.cpu_alloc();
+    d.gpu_alloc();
+    d64.gpu_alloc();
+    load_plummer(h, N, 6, eps * eps);
+    load_plummer(h64, N, 6, eps * eps);
+    // the FP64 run starts from the FP32-rounded state
+    for (int i = 0; i < N; i++) {
+        h64.pos[i] = vec3<double>(h.pos[i].x, h.pos[i].y, h.pos[i].z);
+        h64.vel[i] = vec3<double>(h.vel[i].x, h.vel[i].y, h.vel[i].z);
+        h64.m[i] = h.m[i];
+    }
+    d.host_to_gpu(&h);
+    d64.host_to_gpu(&h64);
+    const double e0 = host_energy(h, N, eps * eps);
+    nbodysystem_globals<float> g32(N, 0.01f, float(eps));
+    nbodysystem_globals<double> g64(N, 0.01, eps);
+    for (int k = 0; k < 4; k++) {
+        hold_step<float, 256>(0, &g32, &d, N, k / 32.0f, (k + 1) / 32.0f, 1.0f / 32, true);
+        hold_step<double, 256>(0, &g64, &d64, N, k / 32.0, (k + 1) / 32.0, 1.0 / 32, true);
+    }
+    out.gpu_to_host(&d);
+    out64.gpu_to_host(&d64);
+    const double de = std::fabs(host_energy(out, N, eps * eps) / e0 - 1.0);
+    double dp = 0.0, rms = 0.0;
+    for (int i = 0; i < N; i++) {
+        const double dx = out.pos[i].x - out64.pos[i].x, dy = out.pos[i].y - out64.pos[i].y,
+                     dz = out.pos[i].z - out64.pos[i].z;
+        dp = std::max(dp, std::sqrt(dx * dx + dy * dy + dz * dz));
+        rms += out64.pos[i].norm2();
+    }
+    rms = std::sqrt(rms / N);
+    CHECK(de < 1e-5, "hold_step<float>: |dE/E| = %.3e", de);
+    CHECK(g32.simtime == 0.125f, "hold_step<float>: simtime %.9g", double(g32.simtime));
+    CHECK(dp <= 1e-5 * rms, "hold_step<float>: max |x32 - x64| = %.3e (rms %.3e)", dp, rms);
+    std::printf("[PASS?] hold_step<float> N=%d |dE/E|=%.3e max|x32-x64|/rms=%.3e\n", N, de, dp / rms);
+    h.cpu_free();
+    out.cpu_free();
+    h64.cpu_free();
+    out64.cpu_free();
+    d.gpu_free();
+    d64.gpu_free();
+}
+
+// hold<T> with the signature the reference declares (src/hold.cuh:20-23):
+// eta and eps from the buffers, the same bits as the globals form, and
+// calc_timestep = false on the first root step reuses the timesteps held.
+void test_hold_declared_signature() {
+    const int N = 300;
+    const double eps = 0.01;
+    particle_system<double> h(N, 'c'), a(N, 'g'), b(N, 'g'), c(N, 'g'), oa(N, 'c'), ob(N, 'c'),
+        oc(N, 'c');
+    h.cpu_alloc();
+    oa.cpu_alloc();
+    ob.cpu_alloc();
+    oc.cpu_alloc();
+    a.gpu_alloc();
+    b.gpu_alloc();
+    c.gpu_alloc();
+    load_plummer(h, N, 8, eps * eps);
+    a.host_to_gpu(&h);
+    b.host_to_gpu(&h);
+    c.host_to_gpu(&h);
+    nbodysystem_buffers<double> buf(N, 0.01, eps);
+    hold<double>(0, &a, &buf, N, 0.0, 0.125, 1.0 / 32, true);
+    nbodysystem_globals<double> g(N, 0.01, eps);
+    hold<double>(0, &g, &b, N, 0.0, 0.125, 1.0 / 32);
+    // timesteps computed up front, then hold without recomputing them first
+    nbodysystem_buffers<double> buf2(N, 0.01, eps);
+    timesteps<double, 256>(&c, buf2.hold_globals(), N);
+    hold<double>(0, &c, &buf2, N, 0.0, 0.125, 1.0 / 32, false);
+    oa.gpu_to_host(&a);
+    ob.gpu_to_host(&b);
+    oc.gpu_to_host(&c);
+    int diff = 0, diffc = 0;
+    for (int i = 0; i < N; i++) {
+        diff += !(oa.pos[i].x == ob.pos[i].x && oa.pos[i].y == ob.pos[i].y &&
+                  oa.pos[i].z == ob.pos[i].z && oa.vel[i].x == ob.vel[i].x &&
+                  oa.vel[i].y == ob.vel[i].y && oa.vel[i].z == ob.vel[i].z);
+        diffc += !(oa.pos[i].x == oc.pos[i].x && oa.vel[i].z == oc.vel[i].z);
+    }
+    CHECK(diff == 0, "hold(buffers): %d particles differ from hold(globals)", diff);
+    CHECK(diffc == 0, "hold(buffers, calc=false after timesteps): %d particles differ", diffc);
+    CHECK(buf.globals && buf.globals->simtime == 0.125, "hold(buffers): simtime");
+    std::printf("[PASS?] hold(clevel, sys, buffers, N, stime, etime, dt, calc) N=%d\n", N);
+    h.cpu_free();
+    oa.cpu_free();
+    ob.cpu_free();
+    oc.cpu_free();
+    a.gpu_free();
+    b.gpu_free();
+    c.gpu_free();
+}
+
 int main() {
     int ndev = 0;
     if (hhs_device_count(&ndev) != HHS_OK || ndev == 0) {
@@ -213,6 +346,8 @@ int main() {
     test_nbforce();
     test_split_update_roundtrip();
     test_hold_step();
+    test_hold_step_f32();
+    test_hold_declared_signature();
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

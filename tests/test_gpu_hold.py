@@ -229,3 +229,31 @@ def test_hold_subtrees_several_root_steps(hhs, oracle, torch_cuda, subtree_max):
     assert np.abs(v - host.vel).max() <= 1e-6 * np.sqrt((host.vel**2).mean())
     mom = (m[:, None] * v).sum(0)
     assert np.abs(mom).max() <= 1e-6 * (m[:, None] * np.abs(v)).sum(0).max()
+
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_hold_fp32_sets_track_oracle(hhs, oracle, torch_cuda, n):
+    """hold_step on FP32 particle sets (the reference's T = float path,
+    src/hold.cuh:51-53): the same recursion as the FP64 CPU HOLD started from
+    the FP32-rounded state, positions and velocities within 1e-5 of the rms
+    (FP32 storage), momentum conserved to FP32 rounding."""
+    import torch
+    eps, eta, dt = 1e-2, 0.01, 1.0 / 32
+    pos, vel, m = plummer(hhs, n, seed=9100 + n, eps2=eps * eps)
+    pos, vel, m = (a.astype(np.float32).astype(np.float64) for a in (pos, vel, m))
+    host = oracle.HostParticles(n, pos, vel, m)
+    rc, ost = oracle.hold(host, dt, dt, eta, eps * eps)
+    assert rc == 0
+    d = hhs.DeviceParticles.from_host(pos, vel, m, dtype=torch.float32)
+    g = hhs.Globals(n, eta, eps)
+    g.hold(d, dt, dt)
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert st["calls"] == ost.calls and st["interactions"] == ost.interactions
+    assert g.simtime == ost.simtime
+    p = d.pos.cpu().numpy().astype(np.float64)
+    v = d.vel.cpu().numpy().astype(np.float64)
+    assert np.abs(p - host.pos).max() <= 1e-5 * np.sqrt((host.pos**2).mean())
+    assert np.abs(v - host.vel).max() <= 1e-5 * np.sqrt((host.vel**2).mean())
+    mom = (m[:, None] * v).sum(0)
+    assert np.abs(mom).max() <= 1e-5 * (m[:, None] * np.abs(v)).sum(0).max()
