@@ -18,12 +18,15 @@ peak.  Beside the headline line the JSON carries:
     0.125, beside its FP64-only run;
   * ensemble -- 4096 Plummer systems (N log-uniform in [8, 2048], t = 1) in
     one persistent launch, against the plain one-system-at-a-time loop;
-  * served -- a Poisson request stream through the continuous-batching
-    server, p50/p99 time-to-solution;
+  * served -- a Poisson request stream (N log-uniform in [64, 65536], t_end
+    uniform in [0.1, 1]) through the continuous-batching server, p50/p99
+    time-to-solution per arrival rate;
+  * split_sweep -- the reference's split timing design: GPU split beside
+    cpu_split at N = 2^10 .. 2^22;
   * cpu_baseline -- the FP64 oracle direct sum on the host, one thread, on
     a bounded sample of sinks, and the kick's force error on that sample;
-  * cpu_reference -- all-threads CPU kick, the reference's cpu_split, and
-    the FP64 CPU HOLD at N = 1024 beside the GPU.
+  * cpu_reference -- all-threads CPU kick, cpu_split at 2^24, and the FP64
+    CPU HOLD at N = 1024 beside the GPU.
 """
 import argparse
 import json
@@ -120,9 +123,8 @@ def box_info():
 
 def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
     """Further CPU timings (steady clock): the oracle direct sum with every
-    host thread; the reference's own cpu_split (src/cpu_scan.h:20-50,
-    compiled from the reference sources into oracle/_ref when present, else
-    the oracle's restatement), median of 5 on 2^24 doubles; and the FP64 CPU
+    host thread; cpu_split (src/cpu_scan.h:20-50, the oracle's restatement),
+    median of 5 on 2^24 floats; and the FP64 CPU
     HOLD at N = 1024 (eps = 0, eta = 0.01, t = 1 in root steps of 1/16)
     beside the GPU FP64-only and default HOLD on the same system."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -143,21 +145,20 @@ def cpu_reference(hhs, torch, pos, m, eps2, kick_seconds=8.0):
     out["kick_all_threads"] = {"value": round(k * N / dt / 1e9, 4), "unit": "Ginteractions/s",
                                "cores": threads, "kind": "port",
                                "sample": f"{k} random sinks x {N} sources"}
-    vals = np.random.default_rng(2).random(1 << 24)
+    vals = (10.0 * np.random.default_rng(2).random(1 << 24)).astype(np.float32)
     times = []
-    use_ref = oracle.ref_available()
     for _ in range(5):
+        arr = vals.copy()
         t0 = time.perf_counter()
-        if use_ref:
-            oracle.ref_cpu_split(vals, 0.5)
-        else:
-            oracle.cpu_split(vals, 0.5)
+        oracle.cpu_split_inplace_f32(arr, 5.0)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     out["cpu_split"] = {"n": 1 << 24, "median_ms": round(med * 1e3, 2),
                         "Melements_per_s": round((1 << 24) / med / 1e6, 1), "cores": 1,
-                        "kind": "reference" if use_ref else "port",
-                        "note": "timing includes one copy of the input array"}
+                        "kind": "port",
+                        "note": "float32 values U[0,10), pivot 5; the oracle's restatement of "
+                                "src/cpu_scan.h:20-50, pinned element for element to the "
+                                "reference's own header in the CPU tests"}
     oracle.set_threads(1)
     p1, v1, m1 = hhs.ic_plummer(1024, 1, 0, True, 0.0)
     e0 = oracle.energy(p1, v1, m1, 0.0)
@@ -366,6 +367,67 @@ def stream_bench(hhs, torch, stream, sizes=(N, 1 << 20, 1 << 22, 1 << 24), reps=
             "peak_GB_per_s": HBM_PEAK_GBS, "sizes": rows}
 
 
+def split_sweep(hhs, torch, stream, log2_sizes=range(10, 23), events=10):
+    """The reference's split timing design (perf/perf_scan_multi_block.cu:186-307):
+    float values U[0,10), pivot 5, N = 2^10 .. 2^22, the GPU split of the value
+    array (hhs_split_values_f32, split.cuh naming: values <= pivot are fast)
+    beside cpu_split (src/cpu_scan.h:20-50; the oracle's restatement, one host
+    thread, in place) on the same values.  GPU: mean of `events` timed calls
+    (HIP events on the launch stream) after one warm-up; CPU: median of
+    `events` calls, each on a fresh copy.  Also the HOLD record split (148 B
+    per particle) at the same sizes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(1)
+    rows = []
+    for k in log2_sizes:
+        n = 1 << k
+        vals = (10.0 * np.random.default_rng(k).random(n)).astype(np.float32)
+        dv = torch.as_tensor(vals).cuda()
+        slow, fast = torch.empty_like(dv), torch.empty_like(dv)
+        part = torch.zeros(2, dtype=torch.int32, device="cuda")
+        ws = hhs.ScanWorkspace(n)
+        hhs.split_values(dv, slow, fast, 5.0, part, ws)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(events):
+            hhs.split_values(dv, slow, fast, 5.0, part, ws)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        g_us = ev0.elapsed_time(ev1) / events * 1e3
+        ns_gpu = int(part[0].item())
+        # the HOLD record split on a random set of the same size
+        tot = hhs.DeviceParticles.empty(n)
+        gen = torch.Generator(device="cuda").manual_seed(k)
+        for f in ("pos", "vel", "m", "timestep"):
+            t = getattr(tot, f)
+            t.copy_(torch.rand(t.shape, generator=gen, dtype=torch.float64, device="cuda"))
+        rs, rf = hhs.DeviceParticles.empty(n), hhs.DeviceParticles.empty(n)
+        hhs.split(tot, rs, rf, 0.5, part, ws)
+        ev0.record(stream)
+        for _ in range(events):
+            hhs.split(tot, rs, rf, 0.5, part, ws)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        r_us = ev0.elapsed_time(ev1) / events * 1e3
+        times = []
+        for _ in range(events):
+            arr = vals.copy()
+            t0 = time.perf_counter()
+            ns_cpu, nf_cpu, _ = oracle.cpu_split_inplace_f32(arr, 5.0)
+            times.append(time.perf_counter() - t0)
+        c_us = float(np.median(times)) * 1e6
+        assert ns_cpu == ns_gpu, (n, ns_cpu, ns_gpu)
+        rows.append({"n": n, "gpu_split_values_us": round(g_us, 2),
+                     "gpu_split_records_us": round(r_us, 2), "cpu_split_us": round(c_us, 2),
+                     "cpu_over_gpu": round(c_us / g_us, 2),
+                     "gpu_records_GB_per_s": round(148 * n / (r_us * 1e-6) / 1e9, 1)})
+        del dv, slow, fast, ws, tot, rs, rf
+    return {"values": "float32 U[0,10), pivot 5", "events": events,
+            "cpu": "cpu_split restated (oracle/hhs_oracle.c), 1 thread, in place",
+            "sizes": rows}
+
+
 def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12, fp64_sample=6):
     """Ensemble workload: 4096 Plummer systems, N log-uniform in [8, 2048],
     equal masses, eta = 0.01, t = 1, eps = 1e-4, root steps of 1/16, FP32
@@ -510,8 +572,12 @@ def main():
                     help="systems timed through the plain per-system loop (0 skips it)")
     ap.add_argument("--fp64-sample", type=int, default=6,
                     help="ensemble systems missing |dE/E| <= 1e-6 rerun by the FP64-only HOLD")
-    ap.add_argument("--serve-full", action="store_true",
-                    help="served stream over the full N range [64, 65536] (long)")
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests per arrival rate in the served leg (N in [64, 65536])")
+    ap.add_argument("--serve-rates", default="1,4",
+                    help="comma-separated Poisson arrival rates (per s) of the served leg")
+    ap.add_argument("--no-split-sweep", action="store_true",
+                    help="skip the GPU split vs cpu_split sweep over N = 2^10 .. 2^22")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="first run one device-to-device copy of MB MiB (a kernel of known "
                          "HBM traffic for scaling profiler byte counters)")
@@ -600,6 +666,9 @@ def main():
     del d
     if not args.no_stream:
         out["stream_kernels"] = stream_bench(hhs, torch, stream)
+    if not args.no_split_sweep and not args.no_cpu:
+        out["split_sweep"] = split_sweep(hhs, torch, stream)
+        log(f"split sweep: {out['split_sweep']}")
     if not args.no_tune:
         out["timestep_autotune"] = tune_timesteps(hhs, torch)
         log(f"timestep autotune: {out['timestep_autotune']}")
@@ -623,12 +692,8 @@ def main():
                                          fp64_sample=args.fp64_sample)
         log(f"ensemble: {out['ensemble']}")
     if not args.no_serve:
-        if args.serve_full:
-            out["served"] = serve_bench(hhs, [0.5, 2.0], 24, 64, 65536, progress=True)
-        else:
-            out["served"] = serve_bench(hhs, [4.0, 16.0], 48, 64, 4096)
-            out["served"]["note"] = ("N range capped at 4096 for the default run; "
-                                     "--serve-full runs [64, 65536]")
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["served"] = serve_bench(hhs, rates, args.serve_requests, 64, 65536, progress=True)
         log(f"served: {out['served']}")
     if not args.no_cpu:
         cb, ferr = cpu_baseline(pos, m, acc_gpu, eps2)

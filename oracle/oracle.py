@@ -188,6 +188,15 @@ def cpu_split_f32(values, pivot):
     return arr, ns.value, nf.value, rc
 
 
+def cpu_split_inplace_f32(arr, pivot):
+    """Restated reference cpu_split on a float32 array, in place (no copy, for
+    timing): (n_slow, n_fast, rc)."""
+    assert arr.dtype == np.float32 and arr.flags.c_contiguous
+    ns, nf = C.c_int(), C.c_int()
+    rc = _lib.oracle_cpu_split_f32(_p(arr), C.c_float(pivot), len(arr), C.byref(ns), C.byref(nf))
+    return ns.value, nf.value, rc
+
+
 def drift(pos, vel, dt, fac):
     pos = _f64(pos).copy()
     vel = _f64(vel)
@@ -247,10 +256,6 @@ def _ref_lib():
             f = getattr(_ref, name)
             f.restype = C.c_int
             f.argtypes = [_vp, ty, _i, _PI, _PI]
-        _ref.ref_vec3_dot.restype = C.c_double
-        _ref.ref_vec3_dot.argtypes = [_vp, _vp]
-        _ref.ref_vec3_norm2.restype = C.c_double
-        _ref.ref_vec3_norm2.argtypes = [_vp]
     return _ref
 
 
@@ -263,13 +268,3 @@ def ref_cpu_split(values, pivot, f32=False):
     fn = lib.ref_cpu_split_f32 if f32 else lib.ref_cpu_split_f64
     fn(_p(arr), pivot, len(arr), C.byref(ns), C.byref(nf))
     return arr, ns.value, nf.value
-
-
-def ref_vec3_dot(a, b):
-    a, b = _f64(a), _f64(b)
-    return _ref_lib().ref_vec3_dot(_p(a), _p(b))
-
-
-def ref_vec3_norm2(a):
-    a = _f64(a)
-    return _ref_lib().ref_vec3_norm2(_p(a))

@@ -257,3 +257,25 @@ def test_hold_fp32_sets_track_oracle(hhs, oracle, torch_cuda, n):
     assert np.abs(v - host.vel).max() <= 1e-5 * np.sqrt((host.vel**2).mean())
     mom = (m[:, None] * v).sum(0)
     assert np.abs(mom).max() <= 1e-5 * (m[:, None] * np.abs(v)).sum(0).max()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "default_host", "default_subtree"])
+def test_hold_calc_timestep_flags_closed_form(hhs, torch_cuda, mode):
+    """The hand-built triple whose call tree follows in closed form (the
+    first recursion reuses the timesteps, the second recomputes them,
+    src/hold.cuh:116-121,168-173): calls per level [1, 2, 2] and 10
+    timestep pairs on the host recursion (FP64-only and default kicks) and
+    in a device subtree."""
+    from conftest import flags_triple
+    pos, vel, m, eta, dt, _ = flags_triple()
+    kw = {"fp64": dict(kick_precision=hhs.KICK_F64_EXACT),
+          "default_host": dict(fast_timesteps=hhs.TIMESTEP_FP32, subtree_max=0),
+          "default_subtree": dict(fast_timesteps=hhs.TIMESTEP_FP32, max_level=40,
+                                  subtree_max=256)}[mode]
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    g = hhs.Globals(3, eta, 0.0, **kw)
+    g.hold_step(d, dt)
+    torch_cuda.cuda.synchronize()
+    st = g.stats()
+    assert st["calls_per_level"] == [1, 2, 2] and st["ts_pairs"] == 10 and st["calls"] == 5
+    assert g.simtime == dt

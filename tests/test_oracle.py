@@ -140,17 +140,6 @@ def test_timestep_min_over_partners_and_softening(oracle):
     assert ts[2] == pytest.approx(tff(4.0 + eps2, 2.0), rel=1e-14)
 
 
-def test_timestep_operation_order_matches_reference_vec3(oracle):
-    if not oracle.ref_available():
-        pytest.skip("reference tree not mounted: oracle/_ref not built")
-    rng = np.random.default_rng(0)
-    for _ in range(200):
-        a, b = rng.standard_normal(3), rng.standard_normal(3)
-        # oracle order: dv.x*dr.x + dv.y*dr.y + dv.z*dr.z and x*x+y*y+z*z
-        assert oracle.ref_vec3_dot(a, b) == (b[0] * a[0] + b[1] * a[1]) + b[2] * a[2]
-        assert oracle.ref_vec3_norm2(a) == (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]
-
-
 # ------------------------------------------------------------ scan/split
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 512, 1023, 2151, 4090])
 def test_exclusive_scan_is_prefix_sum(oracle, n):
@@ -310,3 +299,110 @@ def test_hold_depth_guard(oracle, hhs):
     h = oracle.HostParticles(8, pos, vel, m)
     rc, _ = oracle.hold_step(h, 1.0 / 64, True, 0.01, 0.0, max_level=10)
     assert rc == 5  # ORACLE_ERR_DEPTH == HHS_ERR_DEPTH
+
+
+# ------------------------------------------------ timestep corrections
+# The criterion's corrections (src/timestep.cuh:42, :51) are first-order
+# time derivatives of the raw timesteps along the pair's motion:
+#   free-fall tau = eta/sqrt(2) sqrt(r^3/mu):  dtau/dt = 3 tau u / 2,
+#   fly-by    tau = eta r / |v|:               dtau/dt = tau u (1 + mu/(v^2 r)),
+# with u = (dr.dv)/r^2 and the relative acceleration -mu dr / r^3 (G = 1).
+# The oracle applies tau <- tau / (1 - d/2) with d = dtau/dt (capped at 1),
+# so d = 2 (1 - tau_raw / tau_oracle) can be read back from its output and
+# compared with a central difference of the raw closed form along the true
+# two-body trajectory (scipy's integrator, independent of the formula).  A
+# wrong factor in either correction (3/2, the mu/(v^2 r) term, a sign)
+# misses by O(1).
+
+
+def _two_body_states(r0, v0, mu, times):
+    from scipy.integrate import solve_ivp
+
+    def f(_, y):
+        r = y[:3]
+        return np.concatenate([y[3:], -mu * r / np.linalg.norm(r) ** 3])
+
+    sol = solve_ivp(f, (times[0], times[-1]), np.concatenate([r0, v0]), t_eval=times,
+                    rtol=1e-13, atol=1e-15, method="DOP853")
+    return sol.y[:3].T, sol.y[3:].T
+
+
+def _pair_tau_and_d(oracle, r, v, m1, m2, eta, raw):
+    """Oracle timestep of the pair (r = x_1 - x_2, v = v_1 - v_2) and the
+    correction d read back from it against the raw closed form."""
+    pos = np.array([0.5 * r, -0.5 * r])
+    vel = np.array([0.5 * v, -0.5 * v])
+    ts = oracle.timesteps(pos, vel, np.array([m1, m2]), eta, 0.0)
+    assert ts[0] == ts[1]
+    return ts[0], 2.0 * (1.0 - raw / ts[0])
+
+
+@pytest.mark.parametrize("phase", [0.3, 1.2, 2.2, 3.0, 4.0, 5.5])
+def test_timestep_free_fall_correction_is_time_derivative(oracle, phase):
+    """Bound eccentric orbit (e = 0.6): free-fall is the smaller candidate;
+    the oracle's correction equals d tau_ff / dt along the orbit."""
+    m1, m2, eta = 0.7, 0.3, 1e-3
+    mu = m1 + m2
+    a, e = 1.0, 0.6
+    rp = a * (1 - e)
+    r0, v0 = np.array([rp, 0.0, 0.0]), np.array([0.0, np.sqrt(mu * (1 + e) / rp), 0.1 * 0.0])
+    period = 2 * np.pi * np.sqrt(a ** 3 / mu)
+    t, h = phase / (2 * np.pi) * period, 1e-4 * period
+    R, V = _two_body_states(r0, v0, mu, np.array([0.0, t - h, t, t + h]))
+    raw = [eta / np.sqrt(2) * np.sqrt(np.linalg.norm(x) ** 3 / mu) for x in R[1:]]
+    fb = eta * np.linalg.norm(R[2]) / np.linalg.norm(V[2])
+    assert raw[1] < 0.9 * fb  # the free-fall candidate wins
+    tau, d = _pair_tau_and_d(oracle, R[2], V[2], m1, m2, eta, raw[1])
+    d_true = (raw[2] - raw[0]) / (2 * h)
+    assert abs(d_true) > 1e-5
+    assert d == pytest.approx(d_true, rel=1e-6, abs=1e-12)
+
+
+@pytest.mark.parametrize("t", [-3.0, -1.0, -0.2, 0.3, 1.5, 4.0])
+def test_timestep_fly_by_correction_is_time_derivative(oracle, t):
+    """Hyperbolic encounter (v_inf = 2, impact parameter 0.5): fly-by is the
+    smaller candidate; the oracle's correction equals d tau_fb / dt along the
+    trajectory, including the mu / (v^2 r) term from the gravitational
+    deceleration."""
+    m1, m2, eta = 0.6, 0.4, 1e-3
+    mu = m1 + m2
+    r0, v0 = np.array([-10.0, 0.5, 0.0]), np.array([2.0, 0.0, 0.0])
+    t0 = 5.0  # closest approach near t0 + 0
+    h = 1e-4
+    R, V = _two_body_states(r0, v0, mu, np.array([0.0, t0 + t - h, t0 + t, t0 + t + h]))
+    raw = [eta * np.linalg.norm(x) / np.linalg.norm(v) for x, v in zip(R[1:], V[1:])]
+    ff = eta / np.sqrt(2) * np.sqrt(np.linalg.norm(R[2]) ** 3 / mu)
+    assert raw[1] < 0.9 * ff  # the fly-by candidate wins
+    tau, d = _pair_tau_and_d(oracle, R[2], V[2], m1, m2, eta, raw[1])
+    d_true = (raw[2] - raw[0]) / (2 * h)
+    assert abs(d_true) > 1e-6
+    assert d == pytest.approx(d_true, rel=1e-6, abs=1e-12)
+    # near the encounter the mu / (v^2 r) term is a sizeable part of d: a
+    # correction without it would miss by far more than the tolerance
+    r, v2 = np.linalg.norm(R[2]), np.dot(V[2], V[2])
+    if r < 2.0:
+        assert mu / (v2 * r) > 0.05
+
+
+# ------------------------------------------------ calc_timestep flags
+def test_hold_calc_timestep_flags_closed_form(oracle):
+    """One root step of the triple (src/hold.cuh:74-178).  The root puts the
+    pair in F (tau(0) <= dt) and body 3 in S.  The first recursion (dt/2)
+    must reuse the root's timesteps (calc = false, :116-121): tau(0) > dt/2,
+    a leaf.  The second (:168-173) must recompute them (calc = true) after
+    the pair has moved for dt/2: tau(dt/2) <= dt/2, so it recurses; its own
+    first child reuses tau(dt/2) > dt/4 (a leaf) and its second recomputes
+    tau(3dt/4) > dt/4 (a leaf).  Calls per level [1, 2, 2] and 3*2 + 2 + 2 =
+    10 timestep pairs.  The other flag choices give: swapped [1, 2] and 8;
+    both recompute [1, 2, 2] and 14; neither [1, 2] and 6."""
+    from conftest import flags_triple
+    pos, vel, m, eta, dt, taus = flags_triple()
+    assert dt / 2 < taus[0] <= dt and dt / 4 < taus[1] <= dt / 2 and taus[2] > dt / 4
+    ts = oracle.timesteps(pos, vel, m, eta, 0.0)
+    assert ts[0] == pytest.approx(taus[0], rel=1e-9) and ts[2] > 50 * dt
+    host = oracle.HostParticles(3, pos, vel, m)
+    rc, st = oracle.hold_step(host, dt, True, eta, 0.0)
+    assert rc == 0
+    assert [int(x) for x in st.calls_per_level[:st.max_level + 1]] == [1, 2, 2]
+    assert st.ts_pairs == 10 and st.calls == 5
+    assert st.simtime == dt
