@@ -488,6 +488,8 @@ def ensemble_bench(hhs, torch, n_systems=4096, loop_sample=12, fp64_sample=6):
                           "max": float(de.max())},
             "status_ok": int(ok.sum()),
             "interactions": int(r["interactions"].sum()), "ts_pairs": int(r["ts_pairs"].sum()),
+            "calls": int(r["calls"].sum()),
+            "decisions": "exact (FP32 timesteps within 1e-5 of a pivot recomputed in FP64)",
             "plain_loop": {"systems_sampled": loop_sample,
                            "sample": "evenly spaced in sorted N (stratified)",
                            "seconds_per_system": round(loop_per_system, 4),

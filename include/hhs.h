@@ -241,6 +241,21 @@ int hhs_split_f32(const hhs_particles_f32 *total, hhs_particles_f32 *slow,
                   hhs_particles_f32 *fast, int *predicate, int *scanned, float pivot,
                   int n, hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream);
 
+/* hhs_split_f64 without a workspace argument and without any allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The scan's workspace is a library-owned static device
+ * array of 8256 bytes per GPU (a 64-byte header and one count per chunk,
+ * at most 2048 chunks), which is what a split of any n needs; a larger need
+ * would return HHS_ERR_WORKSPACE.  Calls nothing that allocates.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (different streams, or several host threads) race and must be ordered
+ * by the caller; calls on one stream are ordered by it.  Same results as
+ * hhs_split_f64, bit for bit. */
+int hhs_split_f64_noalloc(const hhs_particles_f64 *total, hhs_particles_f64 *slow,
+                          hhs_particles_f64 *fast, int *predicate, int *scanned, double pivot,
+                          int n, hhs_partition *part_dev, void *stream);
+
 /* Scalar-array split of the reference's scan tests
  * (src/scan_optimized_3.cuh:485-515, split.cuh naming): values <= pivot go
  * to fast, the rest to slow, both stable; no lone-particle rule. */
@@ -446,6 +461,11 @@ typedef struct hhs_ensemble_desc {
                               are written, while the launch still runs; with
                               pos/vel/results/done in host-mapped memory a
                               host thread can finish systems one by one      */
+    int approximate_decisions; /* FP32/FAST timesteps: 0 (default) recomputes
+                              exactly every timestep within the error band of
+                              a pivot, so the slow/fast decisions are the
+                              exact mode's, as in the per-system HOLD; 1
+                              keeps the approximate values                   */
 } hhs_ensemble_desc;
 
 size_t hhs_ensemble_workspace_bytes(int n_systems, long long n_particles);

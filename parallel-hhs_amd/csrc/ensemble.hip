@@ -574,42 +574,46 @@ __device__ __forceinline__ void team_timesteps(const EnsArgs &a, int o, int b, i
         team_timesteps_mode<TEAM, HHS_TIMESTEP_EXACT>(a, jb, je, tile, tm);
 }
 
-// Decision guard (pair_math.h) for the call [jb, je) with time step dt:
-// every timestep within the band of a pivot dt * 2^-k is recomputed with
-// the exact formula over the call's sources, by the wave of the thread that
-// owns the sink (sink i belongs to thread (i - jb) mod TEAM, as in the
-// timestep passes and the partition), 64 sources per step and a wave
-// minimum.  The owner writes the value, so the partition, which reads sink
-// i on that same thread, needs no barrier; a minimum does not depend on
-// order, so the value is the exact mode's bit for bit.
-template <int TEAM>
-__device__ void team_ts_guard(const EnsArgs &a, int jb, int je, double dt, const Team<TEAM> &tm) {
-    const TsGuard g{dt, a.guard_qmax, a.guard_lo, a.guard_hi};
-    const int lane = tm.t & 63;
-    for (int i0 = jb; i0 < je; i0 += TEAM) {
-        const int i = i0 + tm.t;
-        uint64_t mask = __ballot(i < je && guard_near_pivot(g, a.wts[i]));
-        while (mask) {
-            const int l = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-            mask &= mask - 1;
-            const int s = __shfl(i, l, 64);
-            const double xi = a.wpos[3 * s], yi = a.wpos[3 * s + 1], zi = a.wpos[3 * s + 2];
-            const double vxi = a.wvel[3 * s], vyi = a.wvel[3 * s + 1], vzi = a.wvel[3 * s + 2];
-            const double mi = a.wm[s];
-            double best = TsConst<double>::huge;
-            for (int j = jb + lane; j < je; j += 64)
-                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j],
-                                      a.wpos[3 * j + 1], a.wpos[3 * j + 2], a.wvel[3 * j],
-                                      a.wvel[3 * j + 1], a.wvel[3 * j + 2], a.wm[j], a.eta,
-                                      a.c_ff, a.eps2, j != s, best);
+// Decision guard (pair_math.h) on timesteps the partition has just loaded:
+// lane values `ts` of sinks `i` (valid when `in`) of the call whose sources
+// are [jb, je).  guard_mask gives the wave's flagged lanes; guard_fix
+// recomputes each flagged timestep with the exact formula on its wave (64
+// sources per step, a wave minimum) and the owner lane takes the value in
+// its register and stores it, so the partition -- which reads each sink on
+// its owner thread -- needs no barrier.  A minimum does not depend on
+// order, so the value is the exact mode's bit for bit.  Both are called by
+// every lane of the wave.
+__device__ __forceinline__ uint64_t guard_mask(const TsGuard &g, bool in, double ts) {
+    return __ballot(in && guard_near_pivot(g, ts));
+}
+
+__device__ __forceinline__ double guard_fix(const EnsArgs &a, uint64_t mask, int jb, int je,
+                                            int i, double ts) {
+    const int lane = threadIdx.x & 63;
+    while (mask) {
+        const int l = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        mask &= mask - 1;
+        const int s = __shfl(i, l, 64);
+        const double xi = a.wpos[3 * s], yi = a.wpos[3 * s + 1], zi = a.wpos[3 * s + 2];
+        const double vxi = a.wvel[3 * s], vyi = a.wvel[3 * s + 1], vzi = a.wvel[3 * s + 2];
+        const double mi = a.wm[s];
+        double best = TsConst<double>::huge;
+        for (int j = jb + lane; j < je; j += 64)
+            pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, a.wpos[3 * j], a.wpos[3 * j + 1],
+                                  a.wpos[3 * j + 2], a.wvel[3 * j], a.wvel[3 * j + 1],
+                                  a.wvel[3 * j + 2], a.wm[j], a.eta, a.c_ff, a.eps2, j != s,
+                                  best);
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                const double o = __shfl_xor(best, d, 64);
-                best = o < best ? o : best;
-            }
-            if (lane == l) a.wts[s] = best;
+        for (int d = 32; d >= 1; d >>= 1) {
+            const double o = __shfl_xor(best, d, 64);
+            best = o < best ? o : best;
+        }
+        if (lane == l) {
+            ts = best;
+            a.wts[s] = best;
         }
     }
+    return ts;
 }
 
 // Stable partition of [o+b, o+e) by timestep <= pivot, fast first.  Returns
@@ -621,19 +625,35 @@ __device__ void team_ts_guard(const EnsArgs &a, int jb, int je, double dt, const
 // scan's barrier orders every load before every store).  No barrier
 // follows the scan's reads of the wave totals: the caller passes another
 // (the end of the frame's stage) before the next scan.  Larger ranges go
-// through the scratch copy.
-template <int TEAM>
+// through the scratch copy.  guard (calls that have just computed their
+// timesteps in an approximate mode): the decision guard gd runs on the
+// loaded timesteps before any predicate is formed.
+template <int TEAM, bool GUARD>
 __device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivot,
-                              TeamShared<TEAM> &sh,
-                              const Team<TEAM> &tm) {
+                              TeamShared<TEAM> &sh, const Team<TEAM> &tm, bool guard,
+                              TsGuard gd) {
     if (e - b <= TEAM) {
         const int i = o + b + tm.t;
         const bool in = i < o + e;
         const int ic = in ? i : o + b;
-        const double x = a.wpos[3 * ic], y = a.wpos[3 * ic + 1], z = a.wpos[3 * ic + 2];
-        const double vx = a.wvel[3 * ic], vy = a.wvel[3 * ic + 1], vz = a.wvel[3 * ic + 2];
-        const double m = a.wm[ic], ts = a.wts[ic];
-        const int id = a.wid[ic];
+        double x = a.wpos[3 * ic], y = a.wpos[3 * ic + 1], z = a.wpos[3 * ic + 2];
+        double vx = a.wvel[3 * ic], vy = a.wvel[3 * ic + 1], vz = a.wvel[3 * ic + 2];
+        double m = a.wm[ic], ts = a.wts[ic];
+        int id = a.wid[ic];
+        const uint64_t fl = (GUARD && guard) ? guard_mask(gd, in, ts) : 0ull;
+        if (GUARD && fl) {
+            // rare: the record is loaded again after the exact recompute, so
+            // that the loop does not keep the loaded copy in registers
+            ts = guard_fix(a, fl, o + b, o + e, ic, ts);
+            x = a.wpos[3 * ic];
+            y = a.wpos[3 * ic + 1];
+            z = a.wpos[3 * ic + 2];
+            vx = a.wvel[3 * ic];
+            vy = a.wvel[3 * ic + 1];
+            vz = a.wvel[3 * ic + 2];
+            m = a.wm[ic];
+            id = a.wid[ic];
+        }
         const bool p = in && ts <= pivot;
         int excl, nf;
         team_scan_bits<TEAM, false>(p, excl, nf, sh, tm);
@@ -655,7 +675,18 @@ __device__ int team_partition(const EnsArgs &a, int o, int b, int e, double pivo
         return nf;
     }
     int c = 0;
-    for (int i = o + b + tm.t; i < o + e; i += TEAM) c += (a.wts[i] <= pivot) ? 1 : 0;
+    if (GUARD && guard) {
+        for (int c0 = o + b; c0 < o + e; c0 += TEAM) {  // uniform trip count
+            const int i = c0 + tm.t;
+            const bool in = i < o + e;
+            double ts = in ? a.wts[i] : 0.0;
+            const uint64_t fl = guard_mask(gd, in, ts);
+            if (fl) ts = guard_fix(a, fl, o + b, o + e, i, ts);
+            c += (in && ts <= pivot) ? 1 : 0;
+        }
+    } else {
+        for (int i = o + b + tm.t; i < o + e; i += TEAM) c += (a.wts[i] <= pivot) ? 1 : 0;
+    }
     const int nf = team_sum<TEAM>(c, sh, tm);
     if (nf == 1) return 0;               // lone fast particle stays slow, order kept
     if (nf == 0 || nf == e - b) return nf;  // stable partition is the identity
@@ -1063,7 +1094,7 @@ struct SysStats {
 // (so every read of this iteration is done) and ends with one after (so
 // the next iteration reads the new state).  A call without fast particles
 // (a leaf) finishes in its stage 1.
-template <int TEAM, bool SOFT>
+template <int TEAM, bool SOFT, bool GUARD>
 __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int calc,
                                TeamShared<TEAM> &sh, double *tile, const Team<TEAM> &tm,
                                SysStats &st) {
@@ -1092,9 +1123,12 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
                 team_timesteps<TEAM>(a, o, f.b, f.e, tile, tm);
                 st.ts_pairs += static_cast<long long>(f.e - f.b) * (f.e - f.b - 1);
                 tm.sync();
-                if (a.guard) team_ts_guard<TEAM>(a, o + f.b, o + f.e, f.dt, tm);
             }
-            const int nf = team_partition<TEAM>(a, o, f.b, f.e, f.dt, sh, tm);  // :79-83
+            // :79-83, with the decision guard after an approximate timestep
+            // evaluation
+            const TsGuard gd{1.0 / f.dt, a.guard_qmax, a.guard_lo, a.guard_hi};
+            const int nf = team_partition<TEAM, GUARD>(a, o, f.b, f.e, f.dt, sh, tm,
+                                                       f.calc && a.guard, gd);
             if (tm.t == 0) {
                 sh.stack[sp - 1].nf = nf;
                 sh.stack[sp - 1].stage = 1;
@@ -1158,7 +1192,7 @@ __device__ int team_hold_frame(const EnsArgs &a, int o, int n, double dt, int ca
     return status;
 }
 
-template <int TEAM, bool SOFT>
+template <int TEAM, bool SOFT, bool GUARD>
 __device__ void team_run_system(const EnsArgs &a, int s, TeamShared<TEAM> &sh, double *tile,
                                 const Team<TEAM> &tm) {
     const int o = uni(a.offsets[s]);
@@ -1185,7 +1219,7 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared<TEAM> &sh, d
         const double t_end = a.t_end[s], dtr = a.dt_root[s];
         if (!(dtr > 0.0)) status = HHS_ERR_INVALID;
         while (status == HHS_OK && t < t_end) {
-            status = team_hold_frame<TEAM, SOFT>(a, o, n, dtr, 1, sh, tile, tm, st);
+            status = team_hold_frame<TEAM, SOFT, GUARD>(a, o, n, dtr, 1, sh, tile, tm, st);
             if (status == HHS_OK) t += dtr;
         }
         e1 = team_energy<TEAM>(a, o, n, sh, tm);
@@ -1315,7 +1349,7 @@ __global__ __launch_bounds__(256) void rank_items_kernel(EnsArgs a) {
 // and two per CU measured 5-10 % slower.  flatten inlines the whole engine
 // (both team sizes), which the register allocator needs to stay within
 // that budget without spilling.
-template <bool SOFT>
+template <bool SOFT, bool GUARD>
 __global__ __launch_bounds__(kWG, 3) __attribute__((flatten)) void ensemble_kernel(EnsArgs a) {
     __shared__ TeamShared<kWG> sh_wg;
     __shared__ TeamShared<64> sh[kWaves];
@@ -1335,12 +1369,12 @@ __global__ __launch_bounds__(kWG, 3) __attribute__((flatten)) void ensemble_kern
         const int code = a.queue[k];
         if (code >= 0) {
             Team<kWG> tm;
-            team_run_system<kWG, SOFT>(a, code, sh_wg, tiles, tm);
+            team_run_system<kWG, SOFT, GUARD>(a, code, sh_wg, tiles, tm);
         } else {
             const int idx = 4 * (-1 - code) + w;
             if (idx < a.n_small) {
                 Team<64> tw;
-                team_run_system<64, SOFT>(a, a.small_sorted[idx], sh[w],
+                team_run_system<64, SOFT, GUARD>(a, a.small_sorted[idx], sh[w],
                                           tiles + w * 64 * kTsStride, tw);
             }
             __syncthreads();  // the bundle is done before the tiles change hands
@@ -1360,7 +1394,7 @@ __global__ __launch_bounds__(kWG, 3) __attribute__((flatten)) void ensemble_kern
 constexpr int kSubLds = 256;
 static_assert(kSubLds <= kSmallSrc, "an LDS subtree's calls take the quarter-split path");
 
-template <bool SOFT, bool LDS>
+template <bool SOFT, bool LDS, bool GUARD>
 __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void subtree_kernel(
     EnsArgs a, Particles<double> total, int n, int clevel, double dt, int calc, SubtreeAcc *acc) {
     constexpr int TEAM = LDS ? kSubTeam : kWG;
@@ -1395,7 +1429,7 @@ __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void
     for (int l = tm.t; l < kMaxFrames; l += TEAM) sh.level_calls[l] = 0;
     tm.sync();
     SysStats st;
-    const int status = team_hold_frame<TEAM, SOFT>(a, 0, n, dt, calc, sh, tile, tm, st);
+    const int status = team_hold_frame<TEAM, SOFT, GUARD>(a, 0, n, dt, calc, sh, tile, tm, st);
     for (int g = tm.t; g < n; g += TEAM) {
         const int u = a.wid[g];
 #pragma unroll
@@ -1494,10 +1528,18 @@ int launch_subtree(const Particles<double> &total, int n, int clevel, double dt,
     a.ts_mode = ts_mode;
     set_guard(a, ts_mode);
     const bool lds = n <= kSubLds;
-    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula
-    auto kern = static_cast<float>(eps2) > 0.0f
-                    ? (lds ? subtree_kernel<true, true> : subtree_kernel<true, false>)
-                    : (lds ? subtree_kernel<false, true> : subtree_kernel<false, false>);
+    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula.
+    // GUARD: the decision guard of the approximate timestep modes, always on
+    // here, so that a subtree decides as the host recursion does.
+    const bool soft = static_cast<float>(eps2) > 0.0f;
+    auto kern = a.guard ? (soft ? (lds ? subtree_kernel<true, true, true>
+                                       : subtree_kernel<true, false, true>)
+                                : (lds ? subtree_kernel<false, true, true>
+                                       : subtree_kernel<false, false, true>))
+                        : (soft ? (lds ? subtree_kernel<true, true, false>
+                                       : subtree_kernel<true, false, false>)
+                                : (lds ? subtree_kernel<false, true, false>
+                                       : subtree_kernel<false, false, false>));
     hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, clevel, dt,
                        calc, acc_dev);
     HHS_LAUNCH_CHECK("subtree_kernel");
@@ -1598,7 +1640,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
                     d->timestep_mode == HHS_TIMESTEP_FP32,
                 "ensemble: bad timestep_mode %d", d->timestep_mode);
     a.ts_mode = d->timestep_mode;
-    set_guard(a, a.ts_mode);
+    set_guard(a, d->approximate_decisions ? HHS_TIMESTEP_EXACT : a.ts_mode);
 
     // Persistent grid: enough workgroups for every big system and every
     // group of four small ones, capped at 8 per CU.
@@ -1619,10 +1661,16 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     hipLaunchKernelGGL(rank_items_kernel, dim3((a.n_items + 255) / 256), dim3(256), 0, st, a);
     HHS_LAUNCH_CHECK("rank_items_kernel");
     const int grid = std::max(1, std::min(a.n_items, 8 * cus));
-    if (a.eps2f > 0.0f)  // eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula
-        hipLaunchKernelGGL(ensemble_kernel<true>, dim3(grid), dim3(kWG), 0, st, a);
-    else
-        hipLaunchKernelGGL(ensemble_kernel<false>, dim3(grid), dim3(kWG), 0, st, a);
+    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula; GUARD:
+    // the decision guard (approximate timestep modes, unless the descriptor
+    // asks for approximate decisions).  The guard is a template parameter:
+    // compiled into the flattened engine behind a run-time test it slowed
+    // every mode down (MI355X, 4096-system batch: 7.7-8.0 s against 6.1-6.4
+    // s, also in the exact mode where it never runs); as a separate
+    // instantiation both run at 6.2-6.6 s.
+    auto kern = a.eps2f > 0.0f ? (a.guard ? ensemble_kernel<true, true> : ensemble_kernel<true, false>)
+                               : (a.guard ? ensemble_kernel<false, true> : ensemble_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWG), 0, st, a);
     HHS_LAUNCH_CHECK("ensemble_kernel");
     return HHS_OK;
 }

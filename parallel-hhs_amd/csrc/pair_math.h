@@ -210,26 +210,28 @@ constexpr double kGuardBandFp32 = 1e-5;
 constexpr double kGuardBandFast = 1e-12;
 
 struct TsGuard {
-    double dt;                   // the call's dt (the largest pivot)
+    double inv_dt;               // 1 / the call's dt (the largest pivot)
     double qmax;                 // 1 + band
     unsigned long long lo, hi;   // band in units of the FP64 mantissa
 };
 
 inline TsGuard make_ts_guard(double dt, double band) {
     TsGuard g;
-    g.dt = dt;
+    g.inv_dt = 1.0 / dt;
     g.qmax = 1.0 + band;
     g.lo = static_cast<unsigned long long>(band * 4503599627370496.0);            // band * 2^52
     g.hi = static_cast<unsigned long long>((1.0 - 2.0 * band) * 4503599627370496.0);
     return g;
 }
 
-// True when ts / dt lies within the band of a power of two 2^-k, k >= 0:
+// True when q = ts / dt lies within the band of a power of two 2^-k, k >= 0:
 // q = 2^e (1 + f) is within band of 2^e when f <= band and within band of
 // 2^(e+1) when (1 - f) / 2 <= band.  NaN, zero and values above every pivot
-// are never flagged.
+// are never flagged.  q is formed as ts * (1 / dt): exact for the
+// power-of-two dt of HOLD's levels below a power-of-two root step, and
+// within an ulp (far inside the band) otherwise.
 __device__ __forceinline__ bool guard_near_pivot(const TsGuard &g, double ts) {
-    const double q = ts / g.dt;
+    const double q = ts * g.inv_dt;
     if (!(q > 0.0) || q > g.qmax) return false;
     const unsigned long long f =
         static_cast<unsigned long long>(__double_as_longlong(q)) & 0xFFFFFFFFFFFFFull;

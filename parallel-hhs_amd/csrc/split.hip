@@ -26,6 +26,7 @@
 // The lone-fast-particle rule (src/split.cuh:321-324) needs the final count,
 // which only the last chunk knows; a follow-up kernel rewrites the slow set
 // as a plain copy when the raw fast count is 1 (it exits at once otherwise).
+#include <mutex>
 #include <type_traits>
 
 #include "hhs_internal.h"
@@ -76,68 +77,75 @@ __device__ __forceinline__ int block_sum(int x, int *s_red) {
     return t;
 }
 
-// Decision guard of the HOLD split (pair_math.h) over elements [b, e) of
-// the call's set: every timestep within the band of a pivot dt * 2^-k is
+// Decision guard of the HOLD split (pair_math.h) on one tile: thread t holds
+// the timesteps ts[k] of elements base + k * kBlock + t (valid below end) of
+// the call's set.  Every timestep within the band of a pivot dt * 2^-k is
 // recomputed with the exact formula over all n particles of the set (the
-// sources of its timestep evaluation), the workgroup's threads splitting
-// the sources, and written back before any predicate is formed from it.
-// The common case (nothing flagged) costs one compare per element and one
-// barrier per tile.
+// sources of its timestep evaluation), the workgroup's threads splitting the
+// sources, and replaced in its owner's register -- before any predicate is
+// formed from it -- and, with write_back, in set.timestep.  The common case
+// (nothing flagged) costs one compare per element and one barrier.  Called
+// by every thread of the workgroup.
 struct SplitGuard {
     TsGuard g;
     double eta, c_ff, eps2;  // the timestep criterion's parameters
 };
 
-__device__ void guard_chunk(const SplitGuard &gd, const Particles<double> &set, int n, int b,
-                            int e) {
+__device__ void guard_tile(const SplitGuard &gd, const Particles<double> &set, int n, int base,
+                           int end, double (&ts)[kItems], bool write_back) {
     __shared__ int s_list[kTile];
+    __shared__ double s_res[kTile];
     __shared__ int s_cnt;
     __shared__ double s_min[kWaves];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int base = b; base < e; base += kTile) {
-        bool fl[kItems];
-        bool any = false;
+    bool fl[kItems];
+    bool any = false;
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const int i = base + k * kBlock + t;
-            fl[k] = i < e && guard_near_pivot(gd.g, set.timestep[i]);
-            any = any || fl[k];
-        }
-        if (!__syncthreads_or(any)) continue;
-        if (t == 0) s_cnt = 0;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kItems; ++k)
-            if (fl[k]) s_list[atomicAdd(&s_cnt, 1)] = base + k * kBlock + t;
-        __syncthreads();
-        const int cnt = s_cnt;
-        for (int q = 0; q < cnt; ++q) {
-            const int s = s_list[q];
-            const double xi = set.pos[3 * s], yi = set.pos[3 * s + 1], zi = set.pos[3 * s + 2];
-            const double vxi = set.vel[3 * s], vyi = set.vel[3 * s + 1], vzi = set.vel[3 * s + 2];
-            const double mi = set.m[s];
-            double best = TsConst<double>::huge;
-            for (int j = t; j < n; j += kBlock)
-                pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, set.pos[3 * j],
-                                      set.pos[3 * j + 1], set.pos[3 * j + 2], set.vel[3 * j],
-                                      set.vel[3 * j + 1], set.vel[3 * j + 2], set.m[j], gd.eta,
-                                      gd.c_ff, gd.eps2, j != s, best);
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                const double o = __shfl_xor(best, d, 64);
-                best = o < best ? o : best;
-            }
-            if (lane == 0) s_min[wave] = best;
-            __syncthreads();
-            if (t == 0) {
-                double m = s_min[0];
-#pragma unroll
-                for (int w = 1; w < kWaves; ++w) m = s_min[w] < m ? s_min[w] : m;
-                set.timestep[s] = m;
-            }
-            __syncthreads();
-        }
+    for (int k = 0; k < kItems; ++k) {
+        fl[k] = base + k * kBlock + t < end && guard_near_pivot(gd.g, ts[k]);
+        any = any || fl[k];
     }
+    if (!__syncthreads_or(any)) return;
+    if (t == 0) s_cnt = 0;
+    __syncthreads();
+    int slot[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        slot[k] = fl[k] ? atomicAdd(&s_cnt, 1) : -1;
+        if (fl[k]) s_list[slot[k]] = base + k * kBlock + t;
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
+    for (int q = 0; q < cnt; ++q) {
+        const int s = s_list[q];
+        const double xi = set.pos[3 * s], yi = set.pos[3 * s + 1], zi = set.pos[3 * s + 2];
+        const double vxi = set.vel[3 * s], vyi = set.vel[3 * s + 1], vzi = set.vel[3 * s + 2];
+        const double mi = set.m[s];
+        double best = TsConst<double>::huge;
+        for (int j = t; j < n; j += kBlock)
+            pair_timestep<double>(xi, yi, zi, vxi, vyi, vzi, mi, set.pos[3 * j],
+                                  set.pos[3 * j + 1], set.pos[3 * j + 2], set.vel[3 * j],
+                                  set.vel[3 * j + 1], set.vel[3 * j + 2], set.m[j], gd.eta,
+                                  gd.c_ff, gd.eps2, j != s, best);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const double o = __shfl_xor(best, d, 64);
+            best = o < best ? o : best;
+        }
+        if (lane == 0) s_min[wave] = best;
+        __syncthreads();
+        if (t == 0) {
+            double m = s_min[0];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) m = s_min[w] < m ? s_min[w] : m;
+            s_res[q] = m;
+            if (write_back) set.timestep[s] = m;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (fl[k]) ts[k] = s_res[slot[k]];
 }
 
 template <class Op>
@@ -146,11 +154,32 @@ __global__ __launch_bounds__(kBlock) void count_kernel(Op op, int n, int per_chu
     __shared__ int s_red[kWaves];
     const int b = blockIdx.x * per_chunk;
     const int e_end = min(b + per_chunk, n);
-    if constexpr (Op::kGuard)
-        if (op.guard) guard_chunk(op.gd, op.total, n, b, e_end);
     int c = 0;
+    bool guarded = false;
+    if constexpr (Op::kGuard) {
+        if (op.guard) {  // tiles of kItems per thread, the guard on each
+            guarded = true;
+            for (int base = b; base < e_end; base += kTile) {
+                double ts[kItems];
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) {
+                    const int e = base + k * kBlock + static_cast<int>(threadIdx.x);
+                    ts[k] = e < e_end ? op.total.timestep[e] : 0.0;
+                }
+                guard_tile(op.gd, op.total, n, base, e_end, ts, true);
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) {
+                    const int e = base + k * kBlock + static_cast<int>(threadIdx.x);
+                    c += (e < e_end && ts[k] <= op.pivot) ? 1 : 0;
+                }
+            }
+        }
+    }
+    if (!guarded) {
 #pragma unroll 8
-    for (int e = b + static_cast<int>(threadIdx.x); e < e_end; e += kBlock) c += op.value_at(e);
+        for (int e = b + static_cast<int>(threadIdx.x); e < e_end; e += kBlock)
+            c += op.value_at(e);
+    }
     c = block_sum(c, s_red);
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
@@ -168,9 +197,6 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
     const int lane = t & 63;
     const int wave = t >> 6;
     const int g = blockIdx.x;
-    // a single chunk has no count pass: the guard runs here
-    if constexpr (Op::kGuard)
-        if (op.guard && gridDim.x == 1) guard_chunk(op.gd, op.total, n, 0, n);
 
     int offset = 0;
     for (int j = t; j < g; j += kBlock) offset += ws.counts[j];
@@ -184,6 +210,18 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
         for (int k = 0; k < kItems; ++k) {
             const int e = base + k * kBlock + t;
             if (e < c_end) it[k] = op.load(e);
+        }
+        // a single chunk has no count pass: the guard runs on the loaded
+        // records (the emitted timesteps are the refined ones)
+        if constexpr (Op::kGuard) {
+            if (op.guard && gridDim.x == 1) {
+                double ts[kItems];
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) ts[k] = base + k * kBlock + t < c_end ? it[k].ts : 0.0;
+                guard_tile(op.gd, op.total, n, base, c_end, ts, false);
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) it[k].ts = ts[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
@@ -244,10 +282,11 @@ template <typename T> struct SplitRec {
     int id;
 };
 
-template <typename T> struct SplitOp {
-    // the decision guard exists for FP64 sets (HOLD's approximate timestep
-    // modes)
-    static constexpr bool kGuard = std::is_same<T, double>::value;
+// GUARD: the HOLD split's decision guard (FP64 sets after an approximate
+// timestep evaluation), a separate instantiation so that the plain split's
+// kernels carry none of its code.
+template <typename T, bool GUARD = false> struct SplitOp {
+    static constexpr bool kGuard = GUARD && std::is_same<T, double>::value;
     Particles<T> total, slow, fast;
     int *predicate, *scanned;
     T pivot;
@@ -258,7 +297,7 @@ template <typename T> struct SplitOp {
     int drift;
     T h;
     int *raw;
-    // HOLD only: the decision guard, when guard != 0
+    // HOLD only (GUARD): the decision guard, when guard != 0
     int guard;
     SplitGuard gd;
     using Item = SplitRec<T>;
@@ -383,10 +422,10 @@ template <bool BINARY, class Op>
 int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream_t st) {
     const Chunks c = chunks_for(n);
     HHS_REQUIRE(ws_ptr, "scan: null workspace");
-    if (ws_bytes < kScanHeaderBytes + sizeof(int) * static_cast<size_t>(c.count) ||
-        ws_bytes < scan_ws_bytes(n)) {
+    const size_t need = kScanHeaderBytes + sizeof(int) * static_cast<size_t>(c.count);
+    if (ws_bytes < need) {
         set_error("scan: workspace of %zu bytes too small for n=%d (need %zu)", ws_bytes, n,
-                  scan_ws_bytes(n));
+                  need);
         return HHS_ERR_WORKSPACE;
     }
     ScanWs ws{static_cast<uint32_t *>(ws_ptr),
@@ -461,16 +500,19 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       T h_slow, void *ws, size_t ws_bytes, hipStream_t st, double guard_band,
                       double eta, double eps2) {
     HHS_REQUIRE(n > 0 && part && raw, "split_hold: bad arguments");
-    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, raw};
-    if constexpr (SplitOp<T>::kGuard) {
+    if constexpr (std::is_same<T, double>::value) {
         if (guard_band > 0.0) {
+            SplitOp<T, true> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow,
+                                raw};
             op.guard = 1;
             op.gd.g = make_ts_guard(static_cast<double>(pivot), guard_band);
             op.gd.eta = eta;
             op.gd.c_ff = eta / TsConst<double>::sqrt2;
             op.gd.eps2 = eps2;
+            return launch_scan_op<true>(op, n, ws, ws_bytes, st);
         }
     }
+    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, raw};
     return launch_scan_op<true>(op, n, ws, ws_bytes, st);
 }
 
@@ -503,6 +545,38 @@ template int launch_split<double>(const Particles<double> &, const Particles<dou
 template int launch_split<float>(const Particles<float> &, const Particles<float> &,
                                  const Particles<float> &, int *, int *, float, int,
                                  hhs_partition *, void *, size_t, hipStream_t);
+
+// ------------------------------------------------ allocation-free split
+// File-scope device scratch of hhs_split_f64_noalloc: the reduce-then-scan's
+// workspace (the 64-byte header and one count per chunk), whose need is
+// kScanHeaderBytes + 4 * chunks with at most kMaxChunks chunks for any n --
+// 8256 bytes.  Each GPU holds its own copy from the library's load until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: the per-device address comes from hipGetSymbolAddress
+// (noalloc_scratch) and is the only one passed to kernels.
+constexpr size_t kNoAllocScratchBytes = kScanHeaderBytes + sizeof(int) * kMaxChunks;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device (with
+// that device current) on its first call.
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    HHS_HIP(hipGetDevice(&dev));
+    HHS_REQUIRE(dev >= 0 && dev < kMaxDevices, "split_noalloc: device %d beyond %d", dev,
+                kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HHS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return HHS_OK;
+}
 
 }  // namespace hhs
 
@@ -582,6 +656,22 @@ int hhs_split_f64(const hhs_particles_f64 *total, hhs_particles_f64 *slow,
                   hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream) {
     return split_c<double>(total, slow, fast, predicate, scanned, pivot, n, part_dev, ws,
                            ws_bytes, stream);
+}
+
+int hhs_split_f64_noalloc(const hhs_particles_f64 *total, hhs_particles_f64 *slow,
+                          hhs_particles_f64 *fast, int *predicate, int *scanned, double pivot,
+                          int n, hhs_partition *part_dev, void *stream) {
+    HHS_REQUIRE(n >= 0, "split_noalloc: negative n");
+    const size_t need = kScanHeaderBytes + sizeof(int) * static_cast<size_t>(chunks_for(n).count);
+    if (need > kNoAllocScratchBytes) {
+        set_error("split_noalloc: n=%d needs %zu bytes of scratch, the static scratch holds %zu",
+                  n, need, kNoAllocScratchBytes);
+        return HHS_ERR_WORKSPACE;
+    }
+    void *ws = nullptr;
+    if (int rc = noalloc_scratch(&ws)) return rc;
+    return split_c<double>(total, slow, fast, predicate, scanned, pivot, n, part_dev, ws,
+                           kNoAllocScratchBytes, stream);
 }
 
 int hhs_split_f32(const hhs_particles_f32 *total, hhs_particles_f32 *slow,

@@ -80,7 +80,8 @@ class EnsembleDesc(C.Structure):
                 ("dt_root", C.c_void_p), ("eta", C.c_double), ("eps", C.c_double),
                 ("max_level", C.c_int), ("max_calls", C.c_longlong),
                 ("wave_team_max_n", C.c_int), ("results", C.c_void_p),
-                ("timestep_mode", C.c_int), ("done", C.c_void_p)]
+                ("timestep_mode", C.c_int), ("done", C.c_void_p),
+                ("approximate_decisions", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -175,6 +176,7 @@ _SIGS = {
     "hhs_scan_exclusive_i32": (_i, [_vp, _vp, _i, _vp, _sz, _vp]),
     "hhs_split_f64": (_i, [_P64, _P64, _P64, _vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
     "hhs_split_f32": (_i, [_P32, _P32, _P32, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
+    "hhs_split_f64_noalloc": (_i, [_P64, _P64, _P64, _vp, _vp, _d, _i, _vp, _vp]),
     "hhs_split_values_f32": (_i, [_vp, _vp, _vp, _f, _i, _vp, _vp, _sz, _vp]),
     "hhs_split_values_f64": (_i, [_vp, _vp, _vp, _d, _i, _vp, _vp, _sz, _vp]),
     "hhs_drift_f64": (_i, [_vp, _vp, _i, _d, _d, _vp]),
@@ -682,7 +684,7 @@ class Ensemble:
     sequences."""
 
     def __init__(self, systems, t_end, dt_root, eta, eps, max_level=0, max_calls=0,
-                 wave_team_max_n=0, fast_timesteps=False):
+                 wave_team_max_n=0, fast_timesteps=False, exact_decisions=True):
         import numpy as np
         import torch
         self.n_systems = len(systems)
@@ -713,7 +715,7 @@ class Ensemble:
                                  self.vel.data_ptr(), self.m.data_ptr(), self.t_end.data_ptr(),
                                  self.dt_root.data_ptr(), eta, eps, max_level, max_calls,
                                  wave_team_max_n, self.results_dev.data_ptr(),
-                                 _ts_mode(fast_timesteps))
+                                 _ts_mode(fast_timesteps), None, 0 if exact_decisions else 1)
 
     def run(self, stream=None):
         check(_lib.hhs_ensemble_run_f64(C.byref(self.desc),
