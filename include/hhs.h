@@ -184,10 +184,13 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
                       void *stream);
 /* Symmetric evaluation of the same minimum: tau_ij = tau_ji bit for bit, so
  * each unordered pair is evaluated once and feeds both particles (blocks of
- * 1024 particles, partial minima in `ws`, then a min-reduction; no atomics).
- * HHS_TIMESTEP_EXACT stays bit-exact with hhs_timesteps_f64; FP32 is within
- * 1e-6; HHS_TIMESTEP_FAST uses the one-sided kernel.  ws must hold
- * hhs_timesteps_sym_workspace_bytes(n) bytes (ceil(n/1024) * n doubles).
+ * particles, partial minima in `ws`, then a min-reduction; no atomics).
+ * HHS_TIMESTEP_EXACT stays bit-exact with hhs_timesteps_f64; FP32 (from
+ * double-single FP32 copies of the positions and velocities, packed into
+ * `ws`) is within 1e-6; HHS_TIMESTEP_FAST uses the one-sided kernel.  ws must
+ * hold hhs_timesteps_sym_workspace_bytes(n) bytes: the larger of
+ * ceil(n/1024) * n doubles (exact) and 64 n bytes + ceil(n/768) * n floats
+ * (FP32).
  * No reference counterpart (the reference evaluates every ordered pair). */
 size_t hhs_timesteps_sym_workspace_bytes(int n);
 int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,

@@ -155,30 +155,41 @@ __device__ __forceinline__ T pair_tau(T xi, T yi, T zi, T vxi, T vyi, T vzi, T m
     return best;
 }
 
-// FP32 form as the pair's own timestep min(tau_ff, tau_fb), for the
-// symmetric kernel: the arithmetic of pair_timestep_f32 up to the two
-// candidates, each divided by (1 - dtau/2) through the hardware reciprocal
-// (~1 ulp) instead of being folded as a fraction.  NaN candidates are
-// ignored and the result starts at 1e30; symmetric in (i, j) bit for bit.
-// (The one-sided kernels keep the fraction fold: with one sink per lane
-// the cross-multiplied compare is cheaper than two quarter-rate
-// reciprocals, 21.7 vs 24.8 ms at n = 131072; here it is the other way,
-// 15.1 vs 14.3 ms.)  SOFT: eps2 > 0 in FP32, so r^2 > 0 and the r = 0 case
-// needs no test.
+// Double-single (hi, lo) FP32 record of one particle for the symmetric FP32
+// kernel: x, y, z, vx, vy, vz each as hi = (float)v, lo = (float)(v - hi),
+// then m (FP32) -- 13 floats, padded to 16 (64 bytes).
+constexpr int kDsRec = 16;
+
+// x_i - x_j from double-single parts: (hi_i - hi_j) + (lo_i - lo_j), all in
+// FP32.  Relative error a few 2^-24 of the difference (the hi difference is
+// exact for nearby values, by Sterbenz), against half an ulp for the FP64
+// difference rounded once -- 3 FP32 operations instead of an FP64 subtract
+// and a conversion, which on MI355X take ~1.7 FP32 slots each.
+__device__ __forceinline__ float ds_sep(float hi, float lo, float hj, float lj) {
+    return (hi - hj) + (lo - lj);
+}
+
+// FP32 form of the pair's own timestep min(tau_ff, tau_fb) on double-single
+// records (a: the sink in registers, b: the source), for the symmetric
+// kernel: the arithmetic of pair_timestep_f32 from the separations on, each
+// candidate divided by (1 - dtau/2) through the hardware reciprocal (~1 ulp)
+// instead of being folded as a fraction.  NaN candidates are ignored and
+// the result starts at 1e30.  Symmetric in (i, j) bit for bit: a swap flips
+// the sign of every separation exactly (the operations on hi and lo parts
+// are negated operand for operand), and the formula sees them only through
+// r^2, v^2, v.r and m_i + m_j.  SOFT: eps2 > 0 in FP32, so r^2 > 0 and the
+// r = 0 case needs no test.
 template <bool SOFT>
-__device__ __forceinline__ float pair_tau_f32(double xi, double yi, double zi, double vxi,
-                                              double vyi, double vzi, float mi, double xj,
-                                              double yj, double zj, double vxj, double vyj,
-                                              double vzj, float mj, float eta, float c_ff,
-                                              float eps2) {
-    const float drx = static_cast<float>(xi - xj), dry = static_cast<float>(yi - yj),
-                drz = static_cast<float>(zi - zj);
-    const float dvx = static_cast<float>(vxi - vxj), dvy = static_cast<float>(vyi - vyj),
-                dvz = static_cast<float>(vzi - vzj);
+__device__ __forceinline__ float pair_tau_ds(const float (&a)[13], const float *b, float eta,
+                                             float c_ff, float eps2) {
+    const float drx = ds_sep(a[0], a[1], b[0], b[1]), dry = ds_sep(a[2], a[3], b[2], b[3]),
+                drz = ds_sep(a[4], a[5], b[4], b[5]);
+    const float dvx = ds_sep(a[6], a[7], b[6], b[7]), dvy = ds_sep(a[8], a[9], b[8], b[9]),
+                dvz = ds_sep(a[10], a[11], b[10], b[11]);
     const float dr2 = __builtin_fmaf(drx, drx, __builtin_fmaf(dry, dry, __builtin_fmaf(drz, drz, eps2)));
     const float dv2 = __builtin_fmaf(dvx, dvx, __builtin_fmaf(dvy, dvy, dvz * dvz));
     const float rv = __builtin_fmaf(dvx, drx, __builtin_fmaf(dvy, dry, dvz * drz));
-    const float mu = mj + mi;
+    const float mu = b[12] + a[12];
     const float rinv = __builtin_amdgcn_rsqf(dr2);
     const float r = (SOFT || dr2 > 0.0f) ? dr2 * rinv : 0.0f;
     const float rinv2 = rinv * rinv;

@@ -252,23 +252,31 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
 
 namespace {
 // ---------------------------------------------------- symmetric evaluation
-// tau_ij = tau_ji bit for bit (pair_tau), so each unordered pair is
-// evaluated once and feeds the minima of both particles.  Particles are cut
-// into blocks of kSymBlock; workgroup (I, J), I <= J, takes sink block I
-// (kSymSpl sinks per lane, 4 waves) against source block J (one wave-uniform
-// source at a time, scalar loads).  It keeps the row minima of its sinks in
-// registers and reduces each source's column minimum across the wave
-// (shuffles) and the 4 waves (LDS).  Partial minima go to P[K][i], the
-// minimum of particle i over block K, written exactly once:
+// tau_ij = tau_ji bit for bit (pair_tau, pair_tau_ds), so each unordered
+// pair is evaluated once and feeds the minima of both particles.  Particles
+// are cut into blocks of 256 * S (S sinks per lane: 4 in the exact mode, 3
+// in FP32); workgroup (I, J), I <= J, takes sink block I (4 waves) against
+// source block J (one wave-uniform source at a time, scalar loads).  It keeps
+// the row minima of its sinks in registers and reduces each source's column
+// minimum across the wave (shuffles) and the 4 waves (LDS).  Partial minima
+// go to P[K][i], the minimum of particle i over block K, written exactly
+// once:
 //   (I, J), I < J: P[J][i] for i in I (rows), P[I][j] for j in J (columns);
 //   (I, I): P[I][i], all j != i of the block (rows only).
 // A second kernel takes tau_i = min_K P[K][i].  No atomics; the minimum of
 // a set does not depend on the order, so the exact mode stays bit-exact.
-constexpr int kSymSpl = 4;                    // sinks per lane
-constexpr int kSymBlock = 64 * 4 * kSymSpl;   // 1024 particles per block
-
-template <int MODE> struct SymV { using type = double; };
-template <> struct SymV<HHS_TIMESTEP_FP32> { using type = float; };
+//
+// The FP32 kernel reads double-single records (pair_math.h, packed once per
+// launch by pack_ds_kernel): its separations cost 3 FP32 operations instead
+// of an FP64 subtract and a conversion, and a source is one 64-byte scalar
+// load, fetched one iteration ahead.  With 3 sinks per lane (67 VGPRs, 7
+// waves per SIMD) it measured 11.2 ms at n = 131072 on MI355X against 14.0
+// ms for FP64 separations with 4 sinks per lane (5 waves); 3 sinks alone
+// gave 12.3 ms, 2 sinks 13.2 ms.
+constexpr int kSymSpl = 4;                      // exact mode: sinks per lane
+constexpr int kSymBlock = 64 * 4 * kSymSpl;     // 1024 particles per block
+constexpr int kSym32Spl = 3;                    // FP32 mode
+constexpr int kSym32Block = 64 * 4 * kSym32Spl; // 768
 
 template <typename V> __device__ __forceinline__ V vmin(V a, V b) { return b < a ? b : a; }
 
@@ -278,37 +286,21 @@ template <typename V> __device__ __forceinline__ V wave_min(V x) {
     return x;
 }
 
-template <int MODE, bool SOFT>
-__device__ __forceinline__ typename SymV<MODE>::type sym_pair(
-    double xi, double yi, double zi, double vxi, double vyi, double vzi, double mi, float mif,
-    double xj, double yj, double zj, double vxj, double vyj, double vzj, double mj, double eta,
-    double c_ff, double eps2) {
-    if constexpr (MODE == HHS_TIMESTEP_FP32)
-        return pair_tau_f32<SOFT>(xi, yi, zi, vxi, vyi, vzi, mif, xj, yj, zj, vxj, vyj, vzj,
-                            static_cast<float>(mj), static_cast<float>(eta),
-                            static_cast<float>(c_ff), static_cast<float>(eps2));
-    else
-        return pair_tau<double>(xi, yi, zi, vxi, vyi, vzi, mi, xj, yj, zj, vxj, vyj, vzj, mj, eta,
-                                c_ff, eps2);
-}
-
-template <int MODE, bool SOFT>
-__global__ __launch_bounds__(256) void timesteps_sym_kernel(
-    const double *__restrict__ pos, const double *__restrict__ vel, const double *__restrict__ m,
-    int n, double eta, double eps2, typename SymV<MODE>::type *__restrict__ part) {
-    using V = typename SymV<MODE>::type;
+__global__ __launch_bounds__(256) void timesteps_sym_kernel(const double *__restrict__ pos,
+                                                            const double *__restrict__ vel,
+                                                            const double *__restrict__ m, int n,
+                                                            double eta, double eps2,
+                                                            double *__restrict__ part) {
     const int I = blockIdx.x, J = blockIdx.y;
     if (I > J) return;
-    __shared__ V colmin[4][kSymBlock];
+    __shared__ double colmin[4][kSymBlock];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double c_ff = eta / TsConst<double>::sqrt2;
-    const V huge = static_cast<V>(TsConst<double>::huge);
+    const double huge = TsConst<double>::huge;
     double xi[kSymSpl], yi[kSymSpl], zi[kSymSpl], vxi[kSymSpl], vyi[kSymSpl], vzi[kSymSpl],
-        mi[kSymSpl];
-    float mif[kSymSpl];
+        mi[kSymSpl], row[kSymSpl];
     int ii[kSymSpl];
-    V row[kSymSpl];
 #pragma unroll
     for (int s = 0; s < kSymSpl; ++s) {
         ii[s] = I * kSymBlock + s * 256 + wave * 64 + lane;
@@ -320,7 +312,6 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
         vyi[s] = vel[3 * ic + 1];
         vzi[s] = vel[3 * ic + 2];
         mi[s] = m[ic];
-        mif[s] = static_cast<float>(mi[s]);
         row[s] = huge;
     }
     const int j0 = J * kSymBlock;
@@ -332,8 +323,9 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
             const double mj = m[j];
 #pragma unroll
             for (int s = 0; s < kSymSpl; ++s) {
-                const V v = sym_pair<MODE, SOFT>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
-                                           mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
+                const double v = pair_tau<double>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s],
+                                                  mi[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff,
+                                                  eps2);
                 row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
             }
         }
@@ -342,11 +334,12 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
             const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
             const double vxj = vel[3 * j], vyj = vel[3 * j + 1], vzj = vel[3 * j + 2];
             const double mj = m[j];
-            V c = huge;
+            double c = huge;
 #pragma unroll
             for (int s = 0; s < kSymSpl; ++s) {
-                const V v = sym_pair<MODE, SOFT>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s], mi[s],
-                                           mif[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff, eps2);
+                const double v = pair_tau<double>(xi[s], yi[s], zi[s], vxi[s], vyi[s], vzi[s],
+                                                  mi[s], xj, yj, zj, vxj, vyj, vzj, mj, eta, c_ff,
+                                                  eps2);
                 row[s] = vmin(row[s], v);
                 c = vmin(c, v);  // I < J: block I is full, every ii[s] < n
             }
@@ -355,12 +348,103 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(
         }
         __syncthreads();
         for (int jj = threadIdx.x; jj < j1 - j0; jj += 256) {
-            const V c = vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
+            const double c =
+                vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
             part[static_cast<size_t>(I) * n + j0 + jj] = c;  // P[I][j]
         }
     }
 #pragma unroll
     for (int s = 0; s < kSymSpl; ++s)
+        if (ii[s] < n) part[static_cast<size_t>(J) * n + ii[s]] = row[s];  // P[J][i]
+}
+
+// Double-single records of particles [0, n) (pair_math.h, kDsRec floats each).
+__global__ __launch_bounds__(256) void pack_ds_kernel(const double *__restrict__ pos,
+                                                      const double *__restrict__ vel,
+                                                      const double *__restrict__ m, int n,
+                                                      float4 *__restrict__ rec) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float v[kDsRec];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double x = pos[3 * i + k], w = vel[3 * i + k];
+        const float xh = static_cast<float>(x), wh = static_cast<float>(w);
+        v[2 * k] = xh;
+        v[2 * k + 1] = static_cast<float>(x - static_cast<double>(xh));
+        v[6 + 2 * k] = wh;
+        v[6 + 2 * k + 1] = static_cast<float>(w - static_cast<double>(wh));
+    }
+    v[12] = static_cast<float>(m[i]);
+    v[13] = v[14] = v[15] = 0.0f;
+    float4 *o = rec + 4 * static_cast<size_t>(i);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
+template <bool SOFT>
+__global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__restrict__ rec, int n,
+                                                              float eta, float c_ff, float eps2,
+                                                              float *__restrict__ part) {
+    const int I = blockIdx.x, J = blockIdx.y;
+    if (I > J) return;
+    __shared__ float colmin[4][kSym32Block];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const float huge = TsConst<float>::huge;
+    float a[kSym32Spl][13], row[kSym32Spl];
+    int ii[kSym32Spl];
+#pragma unroll
+    for (int s = 0; s < kSym32Spl; ++s) {
+        ii[s] = I * kSym32Block + s * 256 + wave * 64 + lane;
+        const int ic = ii[s] < n ? ii[s] : n - 1;
+        const float *r = rec + kDsRec * static_cast<size_t>(ic);
+#pragma unroll
+        for (int k = 0; k < 13; ++k) a[s][k] = r[k];
+        row[s] = huge;
+    }
+    const int j0 = J * kSym32Block;
+    const int j1 = min(j0 + kSym32Block, n);
+    if (I == J) {
+        for (int j = j0; j < j1; ++j) {
+            const float *b = rec + kDsRec * static_cast<size_t>(j);
+#pragma unroll
+            for (int s = 0; s < kSym32Spl; ++s) {
+                const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
+                row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
+            }
+        }
+    } else {
+        // the source record one iteration ahead (j1 > j0: block J is not empty)
+        float nb[13];
+#pragma unroll
+        for (int k = 0; k < 13; ++k) nb[k] = rec[kDsRec * static_cast<size_t>(j0) + k];
+        for (int j = j0; j < j1; ++j) {
+            float b[13];
+#pragma unroll
+            for (int k = 0; k < 13; ++k) b[k] = nb[k];
+            const int jn = j + 1 < j1 ? j + 1 : j;
+#pragma unroll
+            for (int k = 0; k < 13; ++k) nb[k] = rec[kDsRec * static_cast<size_t>(jn) + k];
+            float c = huge;
+#pragma unroll
+            for (int s = 0; s < kSym32Spl; ++s) {
+                const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
+                row[s] = vmin(row[s], v);
+                c = vmin(c, v);  // I < J: block I is full, every ii[s] < n
+            }
+            c = wave_min(c);
+            if (lane == 0) colmin[wave][j - j0] = c;
+        }
+        __syncthreads();
+        for (int jj = threadIdx.x; jj < j1 - j0; jj += 256) {
+            const float c =
+                vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
+            part[static_cast<size_t>(I) * n + j0 + jj] = c;  // P[I][j]
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kSym32Spl; ++s)
         if (ii[s] < n) part[static_cast<size_t>(J) * n + ii[s]] = row[s];  // P[J][i]
 }
 
@@ -375,12 +459,24 @@ __global__ __launch_bounds__(256) void timesteps_sym_reduce_kernel(const V *__re
     ts[i] = static_cast<double>(best);
 }
 
+// Workspace of the FP32 kernel: the packed records (64 B per particle),
+// then the partial minima (ceil(n / 768) * n floats).
+size_t sym32_rec_bytes(int n) {
+    return (static_cast<size_t>(n) * kDsRec * sizeof(float) + 255) & ~static_cast<size_t>(255);
+}
+size_t sym32_ws_bytes(int n) {
+    const size_t nb = (static_cast<size_t>(n) + kSym32Block - 1) / kSym32Block;
+    return sym32_rec_bytes(n) + nb * static_cast<size_t>(n) * sizeof(float);
+}
+
 }  // namespace
 
 size_t timesteps_sym_ws_bytes(int n) {
     if (n <= 0) return 0;
     const size_t nb = (static_cast<size_t>(n) + kSymBlock - 1) / kSymBlock;
-    return nb * static_cast<size_t>(n) * sizeof(double);
+    const size_t exact = nb * static_cast<size_t>(n) * sizeof(double);
+    const size_t fp32 = sym32_ws_bytes(n);
+    return exact > fp32 ? exact : fp32;
 }
 
 int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
@@ -396,23 +492,31 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
     HHS_REQUIRE(ws && ws_bytes >= timesteps_sym_ws_bytes(n),
                 "timesteps: symmetric workspace of %zu bytes, need %zu", ws_bytes,
                 timesteps_sym_ws_bytes(n));
-    const int nb = (n + kSymBlock - 1) / kSymBlock;
-    const dim3 grid(nb, nb), red((n + 255) / 256);
+    const dim3 red((n + 255) / 256);
     if (mode == HHS_TIMESTEP_FP32) {
-        float *part = static_cast<float *>(ws);
-        if (static_cast<float>(eps2) > 0.0f)
-            hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_FP32, true>), grid, dim3(256), 0,
-                               s, pos, vel, m, n, eta, eps2, part);
+        const int nb = (n + kSym32Block - 1) / kSym32Block;
+        float *rec = static_cast<float *>(ws);
+        float *part = reinterpret_cast<float *>(static_cast<char *>(ws) + sym32_rec_bytes(n));
+        hipLaunchKernelGGL(pack_ds_kernel, red, dim3(256), 0, s, pos, vel, m, n,
+                           reinterpret_cast<float4 *>(rec));
+        HHS_LAUNCH_CHECK("pack_ds_kernel");
+        const float etaf = static_cast<float>(eta);
+        const float cff = static_cast<float>(eta / TsConst<double>::sqrt2);
+        const float e2 = static_cast<float>(eps2);
+        if (e2 > 0.0f)
+            hipLaunchKernelGGL(timesteps_sym32_kernel<true>, dim3(nb, nb), dim3(256), 0, s, rec, n,
+                               etaf, cff, e2, part);
         else
-            hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_FP32, false>), grid, dim3(256),
-                               0, s, pos, vel, m, n, eta, eps2, part);
-        HHS_LAUNCH_CHECK("timesteps_sym_kernel");
+            hipLaunchKernelGGL(timesteps_sym32_kernel<false>, dim3(nb, nb), dim3(256), 0, s, rec,
+                               n, etaf, cff, e2, part);
+        HHS_LAUNCH_CHECK("timesteps_sym32_kernel");
         hipLaunchKernelGGL(timesteps_sym_reduce_kernel<float>, red, dim3(256), 0, s, part, nb, n,
                            ts);
     } else {
+        const int nb = (n + kSymBlock - 1) / kSymBlock;
         double *part = static_cast<double *>(ws);
-        hipLaunchKernelGGL((timesteps_sym_kernel<HHS_TIMESTEP_EXACT, false>), grid, dim3(256), 0,
-                           s, pos, vel, m, n, eta, eps2, part);
+        hipLaunchKernelGGL(timesteps_sym_kernel, dim3(nb, nb), dim3(256), 0, s, pos, vel, m, n,
+                           eta, eps2, part);
         HHS_LAUNCH_CHECK("timesteps_sym_kernel");
         hipLaunchKernelGGL(timesteps_sym_reduce_kernel<double>, red, dim3(256), 0, s, part, nb, n,
                            ts);
