@@ -226,7 +226,8 @@ int hhs_scan_workspace_init(void *ws, size_t bytes, void *stream);
 int hhs_scan_workspace_thread(int n_max, void **ws, size_t *bytes);
 
 /* scan_predicate_{small,large,large_ev} (src/split.cuh:437-565):
- * exclusive prefix sum of n ints, in one pass (decoupled look-back). */
+ * exclusive prefix sum of n ints: one pass (decoupled look-back) up to
+ * 2^18 elements, reduce-then-scan above and for in-place scans (in == out). */
 int hhs_scan_exclusive_i32(const int *in, int *out, int n, void *ws, size_t ws_bytes,
                            void *stream);
 

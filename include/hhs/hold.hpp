@@ -67,7 +67,7 @@ void timesteps(particle_system<T> *sys, nbodysystem_globals<T> *globals, const i
 
 // ------------------------------------------------------------ scan/split
 // scan_predicate_small / _large / _large_ev (src/split.cuh:437-565): all
-// three are the same single-pass exclusive scan here.
+// three are the same exclusive scan here (the library picks its form by n).
 template <int UNROLL>
 void scan_predicate_small(int *predicate, int *scanned_predicate, const int N) {
     size_t bytes = 0;

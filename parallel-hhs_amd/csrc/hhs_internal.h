@@ -133,16 +133,20 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
                          hipStream_t s);
 
 // ----------------------------------------------------------- scan / split
-// Workspace layout: u32 header[16] (header[2] = raw fast count of the last
-// split) followed by one int count per chunk of the reduce-then-scan.
+// Workspace layout: u32 header[16] ([0] look-back epoch, [1] its completion
+// counter, [2] raw fast count of the last split), then one status word per
+// 1024-element tile of the look-back scan (and, for the reduce-then-scan of
+// in-place scans, one int count per chunk after them).
 constexpr size_t kScanHeaderBytes = 64;
 size_t scan_ws_bytes(int n_max);
 int scan_ws_init(void *ws, size_t bytes, hipStream_t s);
 
+// lookback_ws false: a workspace that only the reduce-then-scan uses (the
+// allocation-free split's fixed scratch).
 template <typename T>
 int launch_split(const Particles<T> &total, const Particles<T> &slow, const Particles<T> &fast,
                  int *predicate, int *scanned, T pivot, int n, hhs_partition *part_dev,
-                 void *ws, size_t ws_bytes, hipStream_t s);
+                 void *ws, size_t ws_bytes, hipStream_t s, bool lookback_ws = true);
 
 // ------------------------------------------------------------ drift/update
 template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStream_t s);

@@ -4,28 +4,25 @@
 // (src/split.cuh:107-610).  The reference runs 3-7 launches, 2 cudaMalloc
 // and 2 cudaFree per level plus a blocking cudaDeviceSynchronize
 // (src/split.cuh:430, :513-514, :563-564).  Here the predicate, the scan,
-// the stable scatter and the counts are one reduce-then-scan over large
-// contiguous chunks:
+// the stable scatter and the counts are fused, in one of two forms:
 //
-//  * count_kernel: workgroup g sums the values (predicates) of chunk g and
-//    writes counts[g];
-//  * scan_chunks_kernel: workgroup g adds up counts[0..g) for its offset
-//    (at most kMaxChunks ints, read from L2), then walks its chunk in tiles
-//    of kBlock * kItems elements in index order -- striped loads (element
-//    base + k*kBlock + t), wave ballots / shuffles plus LDS for the tile
-//    scan -- and emits every element with its exclusive prefix; the last
-//    chunk reports the grand total.
-//
-// No inter-workgroup waiting, no device-scope atomics and no state kept
-// between calls: on MI355X a device-scope round trip crosses XCDs
-// (hundreds of ns), which made a chained (decoupled look-back) scan
-// latency-bound.  The cost is one extra read of the predicate source
-// (8 B per particle for the split).  Inputs of at most two tiles take one
-// workgroup and one launch.
+//  * up to kLookbackMaxN elements, a single-pass decoupled look-back scan
+//    (scan_lookback_kernel): one launch, one read of the input; every wait
+//    for a predecessor is bounded, with a fallback that sums the
+//    predecessor's input;
+//  * above, a reduce-then-scan over large contiguous chunks (count_kernel:
+//    workgroup g sums the predicates of chunk g; scan_chunks_kernel:
+//    workgroup g adds up counts[0..g) for its offset and walks its chunk in
+//    tiles of kBlock * kItems elements in index order -- striped loads,
+//    wave ballots / shuffles plus LDS for the tile scan -- emitting every
+//    element with its exclusive prefix; the last chunk reports the grand
+//    total).  No inter-workgroup waiting at all; one extra read of the
+//    predicate source.
 //
 // The lone-fast-particle rule (src/split.cuh:321-324) needs the final count,
-// which only the last chunk knows; a follow-up kernel rewrites the slow set
-// as a plain copy when the raw fast count is 1 (it exits at once otherwise).
+// which only the last tile or chunk knows; a follow-up kernel rewrites the
+// slow set as a plain copy when the raw fast count is 1 (it exits at once
+// otherwise).
 #include <mutex>
 #include <type_traits>
 
@@ -262,6 +259,271 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
     if (g == static_cast<int>(gridDim.x) - 1 && t == 0) op.finish(offset, ws.header);
 }
 
+// ------------------------------------------- single-pass look-back scan
+// Merrill & Garland's decoupled look-back (2016): tile t (kTile elements,
+// blockIdx order) loads its items once, forms its aggregate, publishes it,
+// reads the status words of its predecessors back to the nearest inclusive
+// prefix, publishes its own inclusive prefix and emits its items -- one
+// launch and one read of the input.  Status word of a tile (64 bits): the
+// call's epoch (31 bits), a prefix flag, the 32-bit value (the aggregate,
+// or with the flag the inclusive prefix), stored and loaded with agent-scope
+// atomics.  A word of another epoch is "not yet published".
+//
+// Every wait is bounded: wave 0 polls a window of 64 predecessors (one per
+// lane) at most kSpinLimit times; if a needed predecessor has still not
+// published by then, the whole workgroup computes that tile's aggregate from
+// the input itself (with the decision guard when the op has one, so the value
+// is the one its owner publishes) and the look-back continues below it.  No
+// workgroup ever waits on another without a bound.  (Workgroups are
+// dispatched in blockIdx order, so a predecessor has started when its
+// successor runs and the polls normally end within microseconds.)
+//
+// The epoch lives in the workspace header: every tile reads it at its start
+// and uses epoch + 1; the last tile to finish (a device-scope counter) stores
+// the new epoch and resets the counter, so the next call -- stream-ordered
+// after this one -- sees fresh status.  Before the 31-bit epoch would wrap,
+// the last tile clears every status word, so a stale word never matches.
+//
+// On MI355X it wins while the input is a few hundred tiles (one launch
+// instead of two, no second read): n = 131072 scan 11.7 -> 7.4 us, record
+// split 18.9 -> 16.8 us; n = 262144 split 18.6 -> 14.9 us.  Above that it
+// loses: at 2^24 (16384 tiles, ~2000 resident at once) most tiles find
+// only aggregates in their window and walk back window by window, each poll
+// a cross-XCD atomic round trip, so the prefix frontier serialises: scan
+// 41 -> 637 us, split 0.54 -> 1.02 ms.  Larger inputs therefore take the
+// reduce-then-scan (kLookbackMaxN).
+constexpr int kLookbackMaxN = 1 << 18;
+constexpr int kHdrEpoch = 0, kHdrDone = 1;  // header words; [2] raw fast count
+constexpr uint32_t kEpochMax = (1u << 31) - 1;
+constexpr int kSpinLimit = 4096;
+
+__device__ __forceinline__ uint64_t lb_word(uint32_t epoch, bool prefix, int value) {
+    return (static_cast<uint64_t>(epoch) << 33) | (static_cast<uint64_t>(prefix) << 32) |
+           static_cast<uint32_t>(value);
+}
+__device__ __forceinline__ uint32_t lb_epoch(uint64_t w) { return static_cast<uint32_t>(w >> 33); }
+__device__ __forceinline__ bool lb_prefix(uint64_t w) { return (w >> 32) & 1ull; }
+__device__ __forceinline__ int lb_value(uint64_t w) {
+    return static_cast<int>(static_cast<uint32_t>(w));
+}
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_load(uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the values of tile k (elements [k kTile, min((k+1) kTile, n))) over
+// the workgroup, with the decision guard applied as the tile's owner applies
+// it; every thread gets the sum.  The fallback of the look-back.
+template <class Op>
+__device__ int tile_value_sum(const Op &op, int n, int k, int *s_red) {
+    const int base = k * kTile, end = min(base + kTile, n);
+    const int t = threadIdx.x;
+    int c = 0;
+    if constexpr (Op::kGuard) {
+        if (op.guard) {
+            double ts[kItems];
+#pragma unroll
+            for (int q = 0; q < kItems; ++q) {
+                const int e = base + q * kBlock + t;
+                ts[q] = e < end ? op.total.timestep[e] : 0.0;
+            }
+            guard_tile(op.gd, op.total, n, base, end, ts, false);
+#pragma unroll
+            for (int q = 0; q < kItems; ++q) {
+                const int e = base + q * kBlock + t;
+                c += (e < end && ts[q] <= op.pivot) ? 1 : 0;
+            }
+            return block_sum(c, s_red);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+        const int e = base + q * kBlock + t;
+        c += e < end ? op.value_at(e) : 0;
+    }
+    return block_sum(c, s_red);
+}
+
+// status: one word per tile after the 64-byte header; status_cap words fit
+// the workspace (>= tiles).
+template <bool BINARY, class Op>
+__global__ __launch_bounds__(kBlock) void scan_lookback_kernel(Op op, int n, uint32_t *header,
+                                                               uint64_t *status, int tiles,
+                                                               int status_cap) {
+    __shared__ int s_red[kWaves];
+    __shared__ int s_wave_tot[kItems][kWaves];
+    __shared__ uint32_t s_epoch;
+    __shared__ int s_k, s_acc, s_state, s_last;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    const int tile = blockIdx.x;
+    if (t == 0)
+        s_epoch = __hip_atomic_load(header + kHdrEpoch, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    const int base = tile * kTile;
+    const int c_end = min(base + kTile, n);
+    typename Op::Item it[kItems];
+    int v[kItems], incl[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int e = base + k * kBlock + t;
+        if (e < c_end) it[k] = op.load(e);
+    }
+    if constexpr (Op::kGuard) {
+        if (op.guard) {  // the guard on the loaded records (uniform branch)
+            double ts[kItems];
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) ts[k] = base + k * kBlock + t < c_end ? it[k].ts : 0.0;
+            guard_tile(op.gd, op.total, n, base, c_end, ts, false);
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) it[k].ts = ts[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int e = base + k * kBlock + t;
+        v[k] = e < c_end ? op.value(it[k]) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int x;
+        if constexpr (BINARY) {
+            const uint64_t bal = __ballot(v[k] != 0);
+            const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+            x = __popcll(bal & le);
+        } else {
+            x = wave_inclusive(v[k], lane);
+        }
+        incl[k] = x;
+        if (lane == 63) s_wave_tot[k][wave] = x;
+    }
+    __syncthreads();
+    const uint32_t ep = s_epoch;
+    // in-tile exclusive offsets and the tile's aggregate
+    int local[kItems];
+    int running = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int before = 0, slot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const int c = s_wave_tot[k][w];
+            before += (w < wave) ? c : 0;
+            slot += c;
+        }
+        local[k] = running + before + incl[k] - v[k];
+        running += slot;
+    }
+    const int agg = running;
+    int excl = 0;
+    if (tile == 0) {
+        if (t == 0) lb_store(status, lb_word(ep, true, agg));
+    } else {
+        if (t == 0) {
+            lb_store(status + tile, lb_word(ep, false, agg));
+            s_k = tile - 1;
+            s_acc = 0;
+        }
+        __syncthreads();
+        // Rounds: wave 0 resolves one window (or gives up on it, bounded);
+        // every thread reads the round's outcome into registers and passes a
+        // second barrier before wave 0 may write the next round's.
+        for (;;) {
+            // wave 0: one window of predecessors k, k-1, ..., k-63
+            if (wave == 0) {
+                const int k = s_k;
+                const int idx = k - lane;
+                bool ok = idx < 0;  // below tile 0: never needed (tile 0 is a prefix)
+                uint64_t w = 0;
+                int spins = 0;
+                for (;;) {
+                    if (!ok) {
+                        w = lb_load(status + idx);
+                        ok = lb_epoch(w) == ep;
+                    }
+                    const uint64_t okm = __ballot(ok);
+                    const uint64_t pm = __ballot(ok && idx >= 0 && lb_prefix(w));
+                    const int fp = pm ? __ffsll(static_cast<unsigned long long>(pm)) - 1 : 63;
+                    const uint64_t need = (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
+                    if ((okm & need) == need) {
+                        // lanes 0..fp: aggregates and, at fp (when pm != 0), an
+                        // inclusive prefix; without a prefix the whole window
+                        // is aggregates (tile 0, a prefix, lies further back)
+                        int x = (lane <= fp && idx >= 0) ? lb_value(w) : 0;
+#pragma unroll
+                        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+                        if (lane == 0) {
+                            s_acc += x;
+                            s_k = k - 64;
+                            s_state = pm != 0 ? 1 : 0;  // 1: done, 0: next window
+                        }
+                        break;
+                    }
+                    if (++spins >= kSpinLimit) {
+                        // the nearest needed predecessor that has not published:
+                        // add the aggregates before it; the workgroup sums its
+                        // input, and the look-back resumes below it
+                        const uint64_t miss = need & ~okm;
+                        const int L = __ffsll(static_cast<unsigned long long>(miss)) - 1;
+                        int x = (lane < L) ? lb_value(w) : 0;
+#pragma unroll
+                        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+                        if (lane == 0) {
+                            s_acc += x;
+                            s_k = k - L;  // the tile to sum from its input
+                            s_state = 2;
+                        }
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+            const int state = s_state;
+            const int kk = s_k;
+            __syncthreads();
+            if (state == 1) break;
+            if (state == 2) {
+                const int sum = tile_value_sum(op, n, kk, s_red);
+                if (t == 0) {
+                    s_acc += sum;
+                    s_k = kk - 1;
+                }
+                if (kk == 0) break;  // the fallback summed tile 0
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        excl = s_acc;
+        if (t == 0) lb_store(status + tile, lb_word(ep, true, excl + agg));
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int e = base + k * kBlock + t;
+        if (e < c_end) op.emit(e, it[k], v[k], excl + local[k]);
+    }
+    if (tile == tiles - 1 && t == 0) op.finish(excl + agg, header);
+    // completion: the last tile to finish stores the epoch for the next call
+    if (t == 0) {
+        const uint32_t done = atomicAdd(header + kHdrDone, 1u);
+        s_last = done == static_cast<uint32_t>(tiles - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_last) {
+        if (ep >= kEpochMax)  // before the epoch wraps: no stale word may survive
+            for (int i = t; i < status_cap; i += kBlock) lb_store(status + i, 0ull);
+        __syncthreads();
+        if (t == 0) {
+            __hip_atomic_store(header + kHdrDone, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(header + kHdrEpoch, ep >= kEpochMax ? 0u : ep, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ ops
 
 struct ScanI32Op {
@@ -418,18 +680,42 @@ Chunks chunks_for(int n) {
     return {(tiles + tiles_per - 1) / tiles_per, tiles_per * kTile};
 }
 
+// Workspace use.  The look-back scan (n <= kLookbackMaxN): the 64-byte
+// header, then one status word per tile.  The reduce-then-scan (larger n,
+// in-place scans, and the allocation-free split's fixed scratch,
+// `lookback_ws` false): the header,
+// then one count per chunk -- placed after the status words of a workspace
+// that the look-back also uses, so that no count is ever read as a status
+// word.  A workspace starts zeroed (hhs_scan_workspace_init).
 template <bool BINARY, class Op>
-int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream_t st) {
-    const Chunks c = chunks_for(n);
+int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream_t st,
+                   bool lookback_ws = true, bool in_place = false) {
     HHS_REQUIRE(ws_ptr, "scan: null workspace");
-    const size_t need = kScanHeaderBytes + sizeof(int) * static_cast<size_t>(c.count);
+    uint32_t *header = static_cast<uint32_t *>(ws_ptr);
+    char *after = static_cast<char *>(ws_ptr) + kScanHeaderBytes;
+    const int tiles = (n + kTile - 1) / kTile;
+    const size_t status_bytes = sizeof(uint64_t) * static_cast<size_t>(tiles);
+    if (lookback_ws && !in_place && n <= kLookbackMaxN) {
+        if (ws_bytes < kScanHeaderBytes + status_bytes) {
+            set_error("scan: workspace of %zu bytes too small for n=%d (need %zu)", ws_bytes, n,
+                      kScanHeaderBytes + status_bytes);
+            return HHS_ERR_WORKSPACE;
+        }
+        const int cap = static_cast<int>((ws_bytes - kScanHeaderBytes) / sizeof(uint64_t));
+        hipLaunchKernelGGL((scan_lookback_kernel<BINARY, Op>), dim3(tiles), dim3(kBlock), 0, st, op,
+                           n, header, reinterpret_cast<uint64_t *>(after), tiles, cap);
+        HHS_LAUNCH_CHECK("scan_lookback_kernel");
+        return HHS_OK;
+    }
+    const Chunks c = chunks_for(n);
+    const size_t skip = lookback_ws ? status_bytes : 0;
+    const size_t need = kScanHeaderBytes + skip + sizeof(int) * static_cast<size_t>(c.count);
     if (ws_bytes < need) {
         set_error("scan: workspace of %zu bytes too small for n=%d (need %zu)", ws_bytes, n,
                   need);
         return HHS_ERR_WORKSPACE;
     }
-    ScanWs ws{static_cast<uint32_t *>(ws_ptr),
-              reinterpret_cast<int *>(static_cast<char *>(ws_ptr) + kScanHeaderBytes)};
+    ScanWs ws{header, reinterpret_cast<int *>(after + skip)};
     if (c.count > 1) {
         hipLaunchKernelGGL((count_kernel<Op>), dim3(c.count), dim3(kBlock), 0, st, op, n,
                            c.per_chunk, ws.counts);
@@ -460,7 +746,7 @@ int scan_ws_init(void *ws, size_t bytes, hipStream_t st) {
 template <typename T>
 int launch_split(const Particles<T> &total, const Particles<T> &slow, const Particles<T> &fast,
                  int *predicate, int *scanned, T pivot, int n, hhs_partition *part_dev,
-                 void *ws, size_t ws_bytes, hipStream_t st) {
+                 void *ws, size_t ws_bytes, hipStream_t st, bool lookback_ws) {
     HHS_REQUIRE(n >= 0, "split: negative n");
     HHS_REQUIRE(part_dev, "split: null partition");
     if (n == 0) {
@@ -475,7 +761,7 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
     HHS_REQUIRE(fast.pos && fast.vel && fast.m && fast.timestep && fast.parent_id,
                 "split: null field in fast");
     SplitOp<T> op{total, slow, fast, predicate, scanned, pivot, n, part_dev};
-    int rc = launch_scan_op<true>(op, n, ws, ws_bytes, st);
+    int rc = launch_scan_op<true>(op, n, ws, ws_bytes, st, lookback_ws);
     if (rc) return rc;
     const int fix_groups = (n + 255) / 256 < 512 ? (n + 255) / 256 : 512;
     hipLaunchKernelGGL((lone_fast_fixup_kernel<T>), dim3(fix_groups), dim3(256), 0, st, total,
@@ -541,10 +827,10 @@ template int launch_lone_fast_fixup<float>(const Particles<float> &, const Parti
 
 template int launch_split<double>(const Particles<double> &, const Particles<double> &,
                                   const Particles<double> &, int *, int *, double, int,
-                                  hhs_partition *, void *, size_t, hipStream_t);
+                                  hhs_partition *, void *, size_t, hipStream_t, bool);
 template int launch_split<float>(const Particles<float> &, const Particles<float> &,
                                  const Particles<float> &, int *, int *, float, int,
-                                 hhs_partition *, void *, size_t, hipStream_t);
+                                 hhs_partition *, void *, size_t, hipStream_t, bool);
 
 // ------------------------------------------------ allocation-free split
 // File-scope device scratch of hhs_split_f64_noalloc: the reduce-then-scan's
@@ -584,12 +870,13 @@ using namespace hhs;
 
 template <typename T, typename P>
 static int split_c(const P *total, P *slow, P *fast, int *predicate, int *scanned, T pivot, int n,
-                   hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream) {
+                   hhs_partition *part_dev, void *ws, size_t ws_bytes, void *stream,
+                   bool lookback_ws = true) {
     HHS_REQUIRE(total && slow && fast, "split: null system");
     HHS_REQUIRE(n >= 0 && n <= total->n, "split: n=%d outside [0, %d]", n, total->n);
     HHS_REQUIRE(slow->n >= n && fast->n >= n, "split: slow/fast capacity below n=%d", n);
     return launch_split<T>(from_c(*total), from_c(*slow), from_c(*fast), predicate, scanned,
-                           pivot, n, part_dev, ws, ws_bytes, as_stream(stream));
+                           pivot, n, part_dev, ws, ws_bytes, as_stream(stream), lookback_ws);
 }
 
 template <typename T>
@@ -648,7 +935,10 @@ int hhs_scan_exclusive_i32(const int *in, int *out, int n, void *ws, size_t ws_b
     if (n == 0) return HHS_OK;
     HHS_REQUIRE(in && out, "scan: null array");
     ScanI32Op op{in, out};
-    return launch_scan_op<false>(op, n, ws, ws_bytes, as_stream(stream));
+    // the look-back may sum a predecessor's input after that tile has
+    // written its output: an in-place scan takes the reduce-then-scan
+    const bool in_place = in == out;
+    return launch_scan_op<false>(op, n, ws, ws_bytes, as_stream(stream), true, in_place);
 }
 
 int hhs_split_f64(const hhs_particles_f64 *total, hhs_particles_f64 *slow,
@@ -671,7 +961,7 @@ int hhs_split_f64_noalloc(const hhs_particles_f64 *total, hhs_particles_f64 *slo
     void *ws = nullptr;
     if (int rc = noalloc_scratch(&ws)) return rc;
     return split_c<double>(total, slow, fast, predicate, scanned, pivot, n, part_dev, ws,
-                           kNoAllocScratchBytes, stream);
+                           kNoAllocScratchBytes, stream, false);
 }
 
 int hhs_split_f32(const hhs_particles_f32 *total, hhs_particles_f32 *slow,

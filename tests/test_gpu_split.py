@@ -173,3 +173,41 @@ def test_split_f32(hhs, torch_cuda, ws):
     assert nf == sel.sum()
     assert np.array_equal(fast.parent_id.cpu().numpy()[:nf], np.nonzero(sel)[0])
     assert np.array_equal(slow.parent_id.cpu().numpy()[:ns], np.nonzero(~sel)[0])
+
+
+def test_scan_lookback_epoch_wrap(hhs, torch_cuda):
+    """The look-back scan's status words carry a 31-bit call epoch kept in the
+    workspace header (word 0); before it wraps, the call's last tile clears
+    every status word.  Set the epoch just below the wrap from the host and
+    run scans of several sizes across it: every result stays exact and the
+    epoch restarts at 1.  Scans above 2^18 elements take the reduce-then-scan,
+    whose chunk counts lie beyond every look-back status word: interleaved,
+    they leave the epoch alone."""
+    torch = torch_cuda
+    ws = hhs.ScanWorkspace(1 << 21)
+    hdr = ws.buf[:4].view(torch.int32)
+    hdr.copy_(torch.tensor([2 ** 31 - 3], dtype=torch.int32))
+    torch.cuda.synchronize()
+    for k, n in enumerate([200000, 5000, 700000, 100, 1 << 18, 150001, 1 << 21, 4096]):
+        x = np.random.default_rng(40 + k).integers(0, 2, n).astype(np.int32)
+        inp = torch.as_tensor(x).cuda()
+        out = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+        hhs.scan_exclusive(inp, out, ws)
+        torch.cuda.synchronize()
+        ref = np.concatenate([[0], np.cumsum(x)[:-1]]).astype(np.int32)
+        assert np.array_equal(out.cpu().numpy(), ref), f"call {k} n={n}"
+    # look-back calls 0-1 used epochs 2^31-2 and 2^31-1 (the last one
+    # clears), then calls 3, 4, 5, 7 used 1..4; calls 2 and 6 left it alone
+    assert int(ws.buf[:4].view(torch.int32).cpu()[0]) == 4
+
+
+@pytest.mark.parametrize("n", [1, 1000, 4096, 300001])
+def test_scan_in_place(hhs, torch_cuda, ws, n):
+    """in == out: the in-place scan (reduce-then-scan path) is exact too."""
+    torch = torch_cuda
+    x = np.random.default_rng(n).integers(-50, 50, n).astype(np.int32)
+    buf = torch.as_tensor(x).cuda()
+    hhs.scan_exclusive(buf, buf, ws)
+    torch.cuda.synchronize()
+    ref = np.concatenate([[0], np.cumsum(x.astype(np.int64))[:-1]]).astype(np.int32)
+    assert np.array_equal(buf.cpu().numpy(), ref)
