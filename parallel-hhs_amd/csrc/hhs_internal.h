@@ -18,10 +18,10 @@ void clear_error();
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Per-thread device scratch reused across launches, grown on demand.  A
-// launch on another stream than the previous user first waits (on the
-// device) for that user's completion event, so one host thread's launches
-// on several streams never use it concurrently.  Usage: acquire, queue the
-// kernels on `st`, release(st).
+// launch on another stream than the previous user, or one that regrows it,
+// first waits for the device to drain, so one host thread's launches on
+// several streams never use it concurrently; launches on one stream pay
+// nothing.  Usage: acquire, queue the kernels on `st`, release(st).
 class ThreadScratch {
   public:
     ~ThreadScratch();
@@ -31,7 +31,6 @@ class ThreadScratch {
   private:
     void *ptr_ = nullptr;
     size_t bytes_ = 0;
-    hipEvent_t done_ = nullptr;
     hipStream_t stream_ = nullptr;
     bool used_ = false;
 };
@@ -157,9 +156,10 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
 // (pair_math.h) on the set's timesteps before the predicates are formed.
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
-                      const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
-                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st,
-                      double guard_band = 0.0, double eta = 0.0, double eps2 = 0.0);
+                      const Particles<T> &fast, T pivot, int n, hhs_partition *part,
+                      unsigned long long *ticket, uint32_t seq, T h_slow, void *ws,
+                      size_t ws_bytes, hipStream_t st, double guard_band = 0.0,
+                      double eta = 0.0, double eps2 = 0.0);
 template <typename T>
 int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
                            hipStream_t st);

@@ -14,15 +14,19 @@
 //    rides in the same launch as a second group of sinks; the exact FP64
 //    kick mode keeps the reference's three kicks so its sums follow the
 //    CPU order;
-//  * the only host synchronisation per call is the count read after the
-//    split (the reference blocks at the same point, src/split.cuh:430); the
-//    split writes the counts straight into host-mapped memory;
+//  * the only host wait per call is the count read after the split (the
+//    reference blocks at the same point, src/split.cuh:430): the split
+//    publishes the count in host-mapped memory and the host spins on it,
+//    for a bounded time, instead of synchronising the stream;
 //  * the two slow half-drifts ride on other launches: the first on the
 //    split's scatter of the slow set, the second on the scatter-back (the
 //    fast recursions in between never read the slow set), and both sets
 //    are scattered back by one launch;
 //  * a max_level guard returns HHS_ERR_DEPTH instead of recursing without
 //    bound when timesteps collapse (two coincident particles with eps = 0).
+#include <immintrin.h>
+
+#include <chrono>
 #include <cstring>
 #include <new>
 #include <type_traits>
@@ -58,10 +62,12 @@ struct hhs_globals {
     int *scanned = nullptr;
     void *scan_ws = nullptr;
     size_t scan_ws_bytes = 0;
-    // per level: counts and the raw fast count, in host-mapped pinned memory
+    // per level: the split's ticket (seq << 32 | raw fast count) and its
+    // partition, in host-mapped pinned memory (host / device address)
     void *part_block = nullptr;
-    hhs_partition *part_host = nullptr, *part_map = nullptr;  // host / device address
-    int *raw_host = nullptr, *raw_map = nullptr;
+    unsigned long long *ticket_host = nullptr, *ticket_map = nullptr;
+    hhs_partition *part_host = nullptr, *part_map = nullptr;
+    uint32_t split_seq = 0;  // the last ticket number handed to a split
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
@@ -236,6 +242,45 @@ int call_timesteps(hhs_globals *g, const Particles<float> &total, int n, hipStre
                                    static_cast<float>(g->eta), static_cast<float>(g->eps2), st);
 }
 
+// Ticket numbers of the HOLD splits: nonzero, so that a zeroed ticket word
+// never matches.
+uint32_t next_seq(hhs_globals *g) {
+    if (++g->split_seq == 0) g->split_seq = 1;
+    return g->split_seq;
+}
+
+// The count read-back of src/split.cuh:430.  The split's last workgroup
+// stores the ticket word (seq << 32 | raw fast count) into host-mapped
+// memory with system-scope release; the host spins on it for at most
+// kTicketSpin and then blocks on the stream, which always ends.  Kernels
+// still in flight on the stream need no wait: everything launched next is
+// stream-ordered after them.  On MI355X the spin returns the count ~15 us
+// sooner than a stream synchronisation whose wake-up follows the
+// end-of-kernel signal.
+constexpr auto kTicketSpin = std::chrono::microseconds(2000);
+
+int wait_split(hhs_globals *g, int clevel, uint32_t seq, hipStream_t st, int *raw) {
+    unsigned long long *w = g->ticket_host + clevel;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned i = 1;; ++i) {
+        const unsigned long long v = __atomic_load_n(w, __ATOMIC_ACQUIRE);
+        if (static_cast<uint32_t>(v >> 32) == seq) {
+            *raw = static_cast<int>(static_cast<uint32_t>(v));
+            return HHS_OK;
+        }
+        if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > kTicketSpin) break;
+        _mm_pause();
+    }
+    HHS_HIP(hipStreamSynchronize(st));
+    const unsigned long long v = __atomic_load_n(w, __ATOMIC_ACQUIRE);
+    if (static_cast<uint32_t>(v >> 32) != seq) {
+        set_error("hold_step: the split of level %d published no count", clevel);
+        return HHS_ERR_HIP;
+    }
+    *raw = static_cast<int>(static_cast<uint32_t>(v));
+    return HHS_OK;
+}
+
 template <typename T>
 int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt, bool calc,
              hipStream_t st) {
@@ -267,16 +312,18 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
     // a timestep evaluation in an approximate mode, so that the slow/fast
     // decisions are the exact mode's
     const double band = (f64 && calc) ? guard_band(g) : 0.0;
-    rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel, g->raw_map + clevel,
-                              h, g->scan_ws, g->scan_ws_bytes, st, band, g->eta, g->eps2);
+    const uint32_t seq = next_seq(g);
+    rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel,
+                              g->ticket_map + clevel, seq, h, g->scan_ws, g->scan_ws_bytes, st,
+                              band, g->eta, g->eps2);
     if (rc) return rc;
-    HHS_HIP(hipStreamSynchronize(st));
-    const int ns = g->part_host[clevel].N_slow;
-    const int nf = g->part_host[clevel].N_fast;
+    int raw = 0;
+    if ((rc = wait_split(g, clevel, seq, st, &raw))) return rc;
     // a lone fast particle stays slow (src/split.cuh:321-324): the slow set
     // becomes the whole set, drifted
-    if (g->raw_host[clevel] == 1 && (rc = launch_lone_fast_fixup<T>(total, slow, n, h, st)))
-        return rc;
+    const int nf = raw == 1 ? 0 : raw;
+    const int ns = n - nf;
+    if (raw == 1 && (rc = launch_lone_fast_fixup<T>(total, slow, n, h, st))) return rc;
 
     // src/hold.cuh:106-114
     if (nf == 0) g->simtime += static_cast<double>(dt);
@@ -369,15 +416,17 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
     if (e == hipSuccess) e = hipMalloc(&g->scan_ws, g->scan_ws_bytes);
     const size_t levels = static_cast<size_t>(max_level) + 1;
     if (e == hipSuccess)
-        e = hipHostMalloc(&g->part_block, (sizeof(hhs_partition) + sizeof(int)) * levels,
+        e = hipHostMalloc(&g->part_block,
+                          (sizeof(unsigned long long) + sizeof(hhs_partition)) * levels,
                           hipHostMallocMapped | hipHostMallocCoherent);
     void *map = nullptr;
     if (e == hipSuccess) e = hipHostGetDevicePointer(&map, g->part_block, 0);
     if (e == hipSuccess) {
-        g->part_host = static_cast<hhs_partition *>(g->part_block);
-        g->part_map = static_cast<hhs_partition *>(map);
-        g->raw_host = reinterpret_cast<int *>(g->part_host + levels);
-        g->raw_map = reinterpret_cast<int *>(g->part_map + levels);
+        g->ticket_host = static_cast<unsigned long long *>(g->part_block);
+        g->ticket_map = static_cast<unsigned long long *>(map);
+        g->part_host = reinterpret_cast<hhs_partition *>(g->ticket_host + levels);
+        g->part_map = reinterpret_cast<hhs_partition *>(g->ticket_map + levels);
+        for (size_t l = 0; l < levels; ++l) g->ticket_host[l] = 0;
     }
     if (e == hipSuccess) e = hipMemset(g->scan_ws, 0, g->scan_ws_bytes);
     if (e != hipSuccess) {

@@ -555,10 +555,13 @@ template <typename T, bool GUARD = false> struct SplitOp {
     int n;
     hhs_partition *part;
     // HOLD only: drift the slow records as they are written (pos + vel * h,
-    // the drift kernel's expression) and report the raw fast count.
+    // the drift kernel's expression) and publish the raw fast count in a
+    // host-mapped ticket, seq << 32 | count, stored last with system-scope
+    // release so that a host that sees seq also sees the partition.
     int drift;
     T h;
-    int *raw;
+    unsigned long long *ticket;
+    uint32_t seq;
     // HOLD only (GUARD): the decision guard, when guard != 0
     int guard;
     SplitGuard gd;
@@ -602,7 +605,11 @@ template <typename T, bool GUARD = false> struct SplitOp {
         const int reported = (nf == 1) ? 0 : nf;  // src/split.cuh:321-324
         part->N_fast = reported;
         part->N_slow = n - reported;
-        if (raw) *raw = nf;
+        if (ticket)
+            __hip_atomic_store(ticket,
+                               (static_cast<unsigned long long>(seq) << 32) |
+                                   static_cast<uint32_t>(nf),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 };
 
@@ -782,14 +789,15 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
 // the call's dt and the criterion's eta and eps2, runs in the count pass.
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
-                      const Particles<T> &fast, T pivot, int n, hhs_partition *part, int *raw,
-                      T h_slow, void *ws, size_t ws_bytes, hipStream_t st, double guard_band,
-                      double eta, double eps2) {
-    HHS_REQUIRE(n > 0 && part && raw, "split_hold: bad arguments");
+                      const Particles<T> &fast, T pivot, int n, hhs_partition *part,
+                      unsigned long long *ticket, uint32_t seq, T h_slow, void *ws,
+                      size_t ws_bytes, hipStream_t st, double guard_band, double eta,
+                      double eps2) {
+    HHS_REQUIRE(n > 0 && part && ticket, "split_hold: bad arguments");
     if constexpr (std::is_same<T, double>::value) {
         if (guard_band > 0.0) {
             SplitOp<T, true> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow,
-                                raw};
+                                ticket, seq};
             op.guard = 1;
             op.gd.g = make_ts_guard(static_cast<double>(pivot), guard_band);
             op.gd.eta = eta;
@@ -798,7 +806,7 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
             return launch_scan_op<true>(op, n, ws, ws_bytes, st);
         }
     }
-    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, raw};
+    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, ticket, seq};
     return launch_scan_op<true>(op, n, ws, ws_bytes, st);
 }
 
@@ -814,12 +822,12 @@ int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, 
 
 template int launch_split_hold<double>(const Particles<double> &, const Particles<double> &,
                                        const Particles<double> &, double, int, hhs_partition *,
-                                       int *, double, void *, size_t, hipStream_t, double,
-                                       double, double);
+                                       unsigned long long *, uint32_t, double, void *, size_t,
+                                       hipStream_t, double, double, double);
 template int launch_split_hold<float>(const Particles<float> &, const Particles<float> &,
                                       const Particles<float> &, float, int, hhs_partition *,
-                                      int *, float, void *, size_t, hipStream_t, double, double,
-                                      double);
+                                      unsigned long long *, uint32_t, float, void *, size_t,
+                                      hipStream_t, double, double, double);
 template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
                                             int, double, hipStream_t);
 template int launch_lone_fast_fixup<float>(const Particles<float> &, const Particles<float> &,
