@@ -44,6 +44,13 @@ extern "C" {
 const char *hhs_last_error(void);
 /* Library version string. */
 const char *hhs_version(void);
+/* HHS_OK when one HIP runtime is mapped into the process; HHS_ERR_HIP, with
+ * both paths in hhs_last_error(), when two are (PyTorch-ROCm bundles a
+ * libamdhip64.so.7 of its own: the first device call of a process holding
+ * both crashes).  A host that uses PyTorch loads torch's runtime before this
+ * library, so that libamdhip64.so.7 resolves to it.  No reference
+ * counterpart. */
+int hhs_runtime_check(void);
 
 /* ------------------------------------------------------- data model -- */
 
@@ -336,6 +343,12 @@ int hhs_globals_set_kick_precision(hhs_globals *g, int mode);
  * is HHS_KICK_MIXED32.  Default 256.  No reference counterpart. */
 int hhs_globals_set_subtree_max(hhs_globals *g, int n_max);
 int hhs_globals_subtree_max(const hhs_globals *g);
+/* Speculative launches (default on): the first fast recursion of a call --
+ * its split or its device subtree -- is queued before the host has read the
+ * call's counts, with the set size read on the device; a launch that turns
+ * out not to be the one the recursion makes does nothing.  Results are the
+ * same bits either way.  No reference counterpart. */
+int hhs_globals_set_speculation(hhs_globals *g, int on);
 int hhs_globals_timestep_mode(const hhs_globals *g);
 int hhs_globals_kick_precision(const hhs_globals *g);
 /* Root steps completed by hhs_evolve_f64 (restored from a snapshot on

@@ -1,6 +1,7 @@
 // Error reporting, version and device-memory helpers of the C ABI.
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 
 #include "hhs_internal.h"
 
@@ -64,6 +65,30 @@ extern "C" {
 const char *hhs_last_error(void) { return g_err; }
 
 const char *hhs_version(void) { return "hhs-mi355x 0.1.0 (gfx950)"; }
+
+int hhs_runtime_check(void) {
+    FILE *f = std::fopen("/proc/self/maps", "r");
+    if (!f) return HHS_OK;  // nothing to inspect
+    char line[1024], first[512] = "";
+    int rc = HHS_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        const char *path = std::strchr(line, '/');
+        if (!path || !std::strstr(path, "libamdhip64.so")) continue;
+        char p[512];
+        std::snprintf(p, sizeof p, "%s", path);
+        p[std::strcspn(p, "\n")] = '\0';
+        if (!first[0]) {
+            std::snprintf(first, sizeof first, "%s", p);
+        } else if (std::strcmp(first, p) != 0) {
+            set_error("two HIP runtimes in the process: %s and %s (load PyTorch's first)", first,
+                      p);
+            rc = HHS_ERR_HIP;
+            break;
+        }
+    }
+    std::fclose(f);
+    return rc;
+}
 
 int hhs_device_count(int *count) {
     HHS_REQUIRE(count, "hhs_device_count: null output");

@@ -1396,8 +1396,14 @@ static_assert(kSubLds <= kSmallSrc, "an LDS subtree's calls take the quarter-spl
 
 template <bool SOFT, bool LDS, bool GUARD>
 __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void subtree_kernel(
-    EnsArgs a, Particles<double> total, int n, int clevel, double dt, int calc, SubtreeAcc *acc) {
+    EnsArgs a, Particles<double> total, int n, int clevel, double dt, int calc, SubtreeAcc *acc,
+    const int *n_dev) {
     constexpr int TEAM = LDS ? kSubTeam : kWG;
+    if (n_dev) {  // speculative: the count an earlier kernel wrote, n its bound
+        const int nd = __builtin_amdgcn_readfirstlane(*n_dev);
+        if (nd < 1 || nd > n) return;
+        n = nd;
+    }
     __shared__ TeamShared<TEAM> sh;
     __shared__ double tile[kTileDoubles + kPartDoubles<TEAM>];
     constexpr int C = LDS ? kSubLds : 1;
@@ -1501,9 +1507,11 @@ size_t subtree_workspace_bytes(int n_max) {
 
 int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
                    double eta, double eps2, int ts_mode, int max_level, void *ws,
-                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st) {
+                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st, const int *n_dev) {
     HHS_REQUIRE(n > 0 && clevel >= 0 && max_level >= 0 && max_level <= kMaxFrames - 3 && acc_dev,
                 "subtree: n=%d, clevel=%d, max_level=%d", n, clevel, max_level);
+    // a speculative launch must pick the variant every count up to n picks
+    HHS_REQUIRE(!n_dev || n <= kSubLds, "subtree: a device-side count needs n <= %d", kSubLds);
     const EnsLayout L = layout(1, n);
     HHS_REQUIRE(ws && ws_bytes >= L.total, "subtree: workspace of %zu bytes, need %zu",
                 ws_bytes, L.total);
@@ -1541,7 +1549,7 @@ int launch_subtree(const Particles<double> &total, int n, int clevel, double dt,
                                 : (lds ? subtree_kernel<false, true, false>
                                        : subtree_kernel<false, false, false>));
     hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, clevel, dt,
-                       calc, acc_dev);
+                       calc, acc_dev, n_dev);
     HHS_LAUNCH_CHECK("subtree_kernel");
     return HHS_OK;
 }

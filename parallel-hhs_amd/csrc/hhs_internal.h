@@ -154,12 +154,17 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
 // the scatter-back of both sets with the second slow half-drift.
 // guard_band > 0 runs the decision guard of the approximate timestep modes
 // (pair_math.h) on the set's timesteps before the predicates are formed.
+// fast_count (optional device word) receives the reported fast count.
+// n_dev non-null: a speculative split whose count is read on the device
+// (n is then an upper bound, at most 2^18) and which runs only when that
+// count exceeds n_min.
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       const Particles<T> &fast, T pivot, int n, hhs_partition *part,
-                      unsigned long long *ticket, uint32_t seq, T h_slow, void *ws,
-                      size_t ws_bytes, hipStream_t st, double guard_band = 0.0,
-                      double eta = 0.0, double eps2 = 0.0);
+                      unsigned long long *ticket, uint32_t seq, int *fast_count, T h_slow,
+                      void *ws, size_t ws_bytes, hipStream_t st, double guard_band = 0.0,
+                      double eta = 0.0, double eps2 = 0.0, const int *n_dev = nullptr,
+                      int n_min = 0);
 template <typename T>
 int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
                            hipStream_t st);
@@ -184,9 +189,12 @@ size_t subtree_workspace_bytes(int n_max);
 // hold_step(total[0..n), dt, calc) at level clevel with its whole recursion
 // in one workgroup-wide launch (MIXED32 kicks, timestep mode ts_mode); its
 // statistics are added into acc_dev.  max_level is relative to this call
-// (<= kSubtreeLevels - 3).
+// (<= kSubtreeLevels - 3).  n_dev non-null: a speculative subtree whose
+// count is read on the device (n is then its upper bound, at most 256),
+// which runs only when that count is in [1, n].
 int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
                    double eta, double eps2, int ts_mode, int max_level, void *ws,
-                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st);
+                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st,
+                   const int *n_dev = nullptr);
 
 }  // namespace hhs

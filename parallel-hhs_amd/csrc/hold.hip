@@ -68,6 +68,13 @@ struct hhs_globals {
     unsigned long long *ticket_host = nullptr, *ticket_map = nullptr;
     hhs_partition *part_host = nullptr, *part_map = nullptr;
     uint32_t split_seq = 0;  // the last ticket number handed to a split
+    // device words, per level: the reported fast count of the level's last
+    // split, i.e. the set size of the next level's call (read by the
+    // speculative launches), and the last set size seen per level (the
+    // host's prediction of which launch to speculate)
+    int *next_n_dev = nullptr;
+    int last_n[64];
+    bool speculation = true;
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
@@ -153,9 +160,12 @@ int resolve_kick(const hhs_globals *g) {
 
 // The device subtree runs the MIXED32 kick; other kick modes keep the host
 // recursion so that their arithmetic is the one requested.
+bool subtree_level(const hhs_globals *g, int clevel) {
+    return g->subtree_max > 0 && g->sub_ws && g->active_kick == HHS_KICK_MIXED32 &&
+           g->max_level - clevel <= kSubtreeLevels - 3;
+}
 bool use_subtree(const hhs_globals *g, int n, int clevel) {
-    if (g->subtree_max <= 0 || n > g->subtree_max || !g->sub_ws) return false;
-    return g->active_kick == HHS_KICK_MIXED32 && g->max_level - clevel <= kSubtreeLevels - 3;
+    return n <= g->subtree_max && subtree_level(g, clevel);
 }
 
 // Band of the split's decision guard after a timestep evaluation in the
@@ -170,16 +180,19 @@ double guard_band(const hhs_globals *g) {
 // the host recursion decides depends on it, and its statistics (and a
 // failure) are fetched by fetch_subtree_stats at the end of the hold_step
 // call.
-int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
-                bool calc, hipStream_t st) {
+int queue_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
+                  bool calc, hipStream_t st, const int *n_dev = nullptr) {
     if (!g->sub_pending) {
         HHS_HIP(hipMemsetAsync(g->sub_acc, 0, sizeof(SubtreeAcc), st));
         g->sub_pending = true;
     }
-    int rc = launch_subtree(total, n, clevel, dt, calc ? 1 : 0, g->eta, g->eps2,
-                            g->timestep_mode, g->max_level - clevel, g->sub_ws, g->sub_ws_bytes,
-                            g->sub_acc, st);
-    if (rc) return rc;
+    return launch_subtree(total, n, clevel, dt, calc ? 1 : 0, g->eta, g->eps2, g->timestep_mode,
+                          g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_acc, st,
+                          n_dev);
+}
+int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
+                bool calc, hipStream_t st) {
+    if (int rc = queue_subtree(g, total, n, clevel, dt, calc, st)) return rc;
     g->simtime += dt;  // the leaves of a completed subtree tile its dt
     return HHS_OK;
 }
@@ -281,18 +294,77 @@ int wait_split(hhs_globals *g, int clevel, uint32_t seq, hipStream_t st, int *ra
     return HHS_OK;
 }
 
+// Launches already queued for a call before the host knew its set size.
+struct Queued {
+    bool split = false;    // its split, whose ticket carries `seq`
+    uint32_t seq = 0;
+    bool subtree = false;  // its whole subtree
+};
+
+// Speculation.  The first fast recursion of a call (src/hold.cuh:116-121)
+// starts with no timestep evaluation, so its first launch -- the split of
+// the fast set, or the set's device subtree -- needs nothing but the set
+// size, which the call's own split has just written to next_n_dev.  Queued
+// before the host waits for that split, with the size read on the device,
+// it runs with no host round trip in between.  Each launch checks the size
+// itself and does nothing unless the host recursion would make that same
+// launch for it (the subtree: size in [1, subtree_max]; the split: above
+// that, or above 0 where no subtree runs), so the host only has to skip it
+// later.  The launch the level's last set size predicts is queued, both when
+// there is none yet.  FP64 subtrees need subtree_max <= 256 (one variant for
+// every size); splits need the call's size, their bound, <= 2^18.
+template <typename T>
+int speculate(hhs_globals *g, const Particles<T> &set, int n_bound, int clevel, T dt,
+              hipStream_t st, Queued *q) {
+    *q = Queued{};
+    if (!g->speculation || clevel > g->max_level || clevel >= 64) return HHS_OK;
+    constexpr bool f64 = std::is_same<T, double>::value;
+    const bool sub = f64 && subtree_level(g, clevel) && g->subtree_max <= 256;
+    const int n_min = (f64 && subtree_level(g, clevel)) ? g->subtree_max : 0;
+    const int last = g->last_n[clevel];
+    const bool want_sub = sub && (last < 0 || last <= g->subtree_max);
+    const bool want_split = n_bound > n_min && n_bound <= (1 << 18) && (last < 0 || last > n_min);
+    const int *n_dev = g->next_n_dev + clevel;
+    int rc;
+    if constexpr (f64) {
+        if (want_sub) {
+            if ((rc = queue_subtree(g, set, g->subtree_max, clevel, dt, false, st, n_dev)))
+                return rc;
+            q->subtree = true;
+        }
+    }
+    if (want_split) {
+        if ((rc = ensure_level(g, clevel))) return rc;
+        const LevelBuffers &lb = g->levels[clevel];
+        q->seq = next_seq(g);
+        rc = launch_split_hold<T>(set, lb.slow_set<T>(), lb.fast_set<T>(), dt, n_bound,
+                                  g->part_map + clevel, g->ticket_map + clevel, q->seq,
+                                  g->next_n_dev + clevel + 1, dt * T(0.5), g->scan_ws,
+                                  g->scan_ws_bytes, st, 0.0, 0.0, 0.0, n_dev, n_min);
+        if (rc) return rc;
+        q->split = true;
+    }
+    return HHS_OK;
+}
+
 template <typename T>
 int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt, bool calc,
-             hipStream_t st) {
+             hipStream_t st, Queued queued = Queued{}) {
     constexpr bool f64 = std::is_same<T, double>::value;
     if (clevel > g->max_level) {
         set_error("hold_step: recursion level %d exceeds max_level %d (timesteps collapsed?)",
                   clevel, g->max_level);
         return HHS_ERR_DEPTH;
     }
+    if (clevel < 64) g->last_n[clevel] = n;
     hhs_hold_stats &S = g->stats;
-    if constexpr (f64)
-        if (use_subtree(g, n, clevel)) return run_subtree(g, total, n, clevel, dt, calc, st);
+    if constexpr (f64) {
+        if (use_subtree(g, n, clevel)) {
+            if (!queued.subtree) return run_subtree(g, total, n, clevel, dt, calc, st);
+            g->simtime += dt;  // queued by the parent, and it ran with this n
+            return HHS_OK;
+        }
+    }
     S.calls++;
     if (clevel < 64) S.calls_per_level[clevel]++;
     if (clevel > S.max_level) S.max_level = clevel;
@@ -310,13 +382,19 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
     // src/hold.cuh:79-83 with the first slow half-drift, then the count
     // read-back (src/split.cuh:430); FP64 sets run the decision guard after
     // a timestep evaluation in an approximate mode, so that the slow/fast
-    // decisions are the exact mode's
-    const double band = (f64 && calc) ? guard_band(g) : 0.0;
-    const uint32_t seq = next_seq(g);
-    rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel,
-                              g->ticket_map + clevel, seq, h, g->scan_ws, g->scan_ws_bytes, st,
-                              band, g->eta, g->eps2);
-    if (rc) return rc;
+    // decisions are the exact mode's.  A split the parent queued has run
+    // (or is running) already.
+    uint32_t seq = queued.seq;
+    if (!queued.split) {
+        const double band = (f64 && calc) ? guard_band(g) : 0.0;
+        seq = next_seq(g);
+        rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel,
+                                  g->ticket_map + clevel, seq, g->next_n_dev + clevel + 1, h,
+                                  g->scan_ws, g->scan_ws_bytes, st, band, g->eta, g->eps2);
+        if (rc) return rc;
+    }
+    Queued child;
+    if ((rc = speculate<T>(g, fast, n, clevel + 1, dt / T(2), st, &child))) return rc;
     int raw = 0;
     if ((rc = wait_split(g, clevel, seq, st, &raw))) return rc;
     // a lone fast particle stays slow (src/split.cuh:321-324): the slow set
@@ -328,7 +406,8 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
     // src/hold.cuh:106-114
     if (nf == 0) g->simtime += static_cast<double>(dt);
     // src/hold.cuh:116-121
-    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), false, st))) return rc;
+    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), false, st, child)))
+        return rc;
     if (ns > 0) {
         // src/hold.cuh:123-124: done by the split
         // src/hold.cuh:137-145: slow sinks <- {slow, fast}; fast sinks <- slow.
@@ -428,6 +507,10 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
         g->part_map = reinterpret_cast<hhs_partition *>(g->ticket_map + levels);
         for (size_t l = 0; l < levels; ++l) g->ticket_host[l] = 0;
     }
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&g->next_n_dev), sizeof(int) * (levels + 1));
+    if (e == hipSuccess) e = hipMemset(g->next_n_dev, 0, sizeof(int) * (levels + 1));
+    for (int &v : g->last_n) v = -1;
     if (e == hipSuccess) e = hipMemset(g->scan_ws, 0, g->scan_ws_bytes);
     if (e != hipSuccess) {
         int rc = hip_status(e, "globals_create", __FILE__, __LINE__);
@@ -460,6 +543,7 @@ int hhs_globals_destroy(hhs_globals *g) {
     if (g->predicate) (void)hipFree(g->predicate);
     if (g->scanned) (void)hipFree(g->scanned);
     if (g->scan_ws) (void)hipFree(g->scan_ws);
+    if (g->next_n_dev) (void)hipFree(g->next_n_dev);
     if (g->part_block) (void)hipHostFree(g->part_block);
     if (g->sub_ws) (void)hipFree(g->sub_ws);
     if (g->ts_ws) (void)hipFree(g->ts_ws);
@@ -502,6 +586,11 @@ int hhs_globals_set_subtree_max(hhs_globals *g, int n_max) {
     return alloc_subtree(g, n_max);
 }
 int hhs_globals_subtree_max(const hhs_globals *g) { return g ? g->subtree_max : 0; }
+int hhs_globals_set_speculation(hhs_globals *g, int on) {
+    HHS_REQUIRE(g, "set_speculation: null globals");
+    g->speculation = on != 0;
+    return HHS_OK;
+}
 
 int hhs_globals_timestep_mode(const hhs_globals *g) { return g ? g->timestep_mode : -1; }
 int hhs_globals_kick_precision(const hhs_globals *g) { return g ? g->kick_precision : -1; }

@@ -183,8 +183,8 @@ __global__ __launch_bounds__(kBlock) void count_kernel(Op op, int n, int per_chu
 
 // Op supplies value_at(e) (the int summed), an Item type with load(e) (every
 // field the element needs, so all of a tile's loads are in flight before any
-// store), value(item), emit(e, item, v, excl), and finish(total, header)
-// (called once, by the last chunk, with the grand total).
+// store), value(item), emit(e, item, v, excl), and finish(total, n, header)
+// (called once, by the last chunk, with the grand total and the count).
 template <bool BINARY, class Op>
 __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int per_chunk,
                                                              ScanWs ws) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
         offset += running;
         __syncthreads();  // s_wave_tot is rewritten by the next tile
     }
-    if (g == static_cast<int>(gridDim.x) - 1 && t == 0) op.finish(offset, ws.header);
+    if (g == static_cast<int>(gridDim.x) - 1 && t == 0) op.finish(offset, n, ws.header);
 }
 
 // ------------------------------------------- single-pass look-back scan
@@ -347,11 +347,16 @@ __device__ int tile_value_sum(const Op &op, int n, int k, int *s_red) {
 }
 
 // status: one word per tile after the 64-byte header; status_cap words fit
-// the workspace (>= tiles).
+// the workspace (>= tiles).  n_dev non-null: the element count is read from
+// device memory (a count an earlier kernel on the stream wrote), the grid
+// covers an upper bound of it, and the launch does nothing at all unless
+// the count exceeds n_min; workgroups past the count's tiles exit at once
+// and take no part in the protocol.
 template <bool BINARY, class Op>
 __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(Op op, int n, uint32_t *header,
                                                                uint64_t *status, int tiles,
-                                                               int status_cap) {
+                                                               int status_cap,
+                                                               const int *n_dev, int n_min) {
     __shared__ int s_red[kWaves];
     __shared__ int s_wave_tot[kItems][kWaves];
     __shared__ uint32_t s_epoch;
@@ -360,6 +365,13 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(Op op, int n, uin
     const int lane = t & 63;
     const int wave = t >> 6;
     const int tile = blockIdx.x;
+    if (n_dev) {  // n is the bound the grid covers
+        const int nd = __builtin_amdgcn_readfirstlane(*n_dev);
+        if (nd <= n_min || nd > n) return;
+        n = nd;
+        tiles = (n + kTile - 1) / kTile;
+        if (tile >= tiles) return;
+    }
     if (t == 0)
         s_epoch = __hip_atomic_load(header + kHdrEpoch, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -505,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(Op op, int n, uin
         const int e = base + k * kBlock + t;
         if (e < c_end) op.emit(e, it[k], v[k], excl + local[k]);
     }
-    if (tile == tiles - 1 && t == 0) op.finish(excl + agg, header);
+    if (tile == tiles - 1 && t == 0) op.finish(excl + agg, n, header);
     // completion: the last tile to finish stores the epoch for the next call
     if (t == 0) {
         const uint32_t done = atomicAdd(header + kHdrDone, 1u);
@@ -535,7 +547,7 @@ struct ScanI32Op {
     __device__ Item load(int e) const { return in[e]; }
     __device__ int value(Item x) const { return x; }
     __device__ void emit(int e, Item, int, int excl) const { out[e] = excl; }
-    __device__ void finish(int, uint32_t *) const {}
+    __device__ void finish(int, int, uint32_t *) const {}
 };
 
 // One particle record as read by the split (68 B in FP64).
@@ -562,6 +574,7 @@ template <typename T, bool GUARD = false> struct SplitOp {
     T h;
     unsigned long long *ticket;
     uint32_t seq;
+    int *fast_count;  // device word: the reported fast count, the next level's n
     // HOLD only (GUARD): the decision guard, when guard != 0
     int guard;
     SplitGuard gd;
@@ -600,11 +613,12 @@ template <typename T, bool GUARD = false> struct SplitOp {
         if (d.id) d.id[o] = r.id;
         d.parent_id[o] = e;
     }
-    __device__ void finish(int nf, uint32_t *header) const {
+    __device__ void finish(int nf, int n, uint32_t *header) const {
         header[2] = static_cast<uint32_t>(nf);
         const int reported = (nf == 1) ? 0 : nf;  // src/split.cuh:321-324
         part->N_fast = reported;
         part->N_slow = n - reported;
+        if (fast_count) *fast_count = reported;
         if (ticket)
             __hip_atomic_store(ticket,
                                (static_cast<unsigned long long>(seq) << 32) |
@@ -630,7 +644,7 @@ template <typename T> struct SplitValuesOp {
         else
             slow[e - f] = x;
     }
-    __device__ void finish(int nf, uint32_t *header) const {
+    __device__ void finish(int nf, int n, uint32_t *header) const {
         header[2] = static_cast<uint32_t>(nf);
         part->N_fast = nf;
         part->N_slow = n - nf;
@@ -694,10 +708,15 @@ Chunks chunks_for(int n) {
 // then one count per chunk -- placed after the status words of a workspace
 // that the look-back also uses, so that no count is ever read as a status
 // word.  A workspace starts zeroed (hhs_scan_workspace_init).
+// n_dev non-null: n is an upper bound of the count read on the device (the
+// look-back form only), and nothing runs unless that count exceeds n_min.
 template <bool BINARY, class Op>
 int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream_t st,
-                   bool lookback_ws = true, bool in_place = false) {
+                   bool lookback_ws = true, bool in_place = false, const int *n_dev = nullptr,
+                   int n_min = 0) {
     HHS_REQUIRE(ws_ptr, "scan: null workspace");
+    HHS_REQUIRE(!n_dev || (lookback_ws && !in_place && n <= kLookbackMaxN),
+                "scan: a device-side count needs the look-back form (n <= %d)", kLookbackMaxN);
     uint32_t *header = static_cast<uint32_t *>(ws_ptr);
     char *after = static_cast<char *>(ws_ptr) + kScanHeaderBytes;
     const int tiles = (n + kTile - 1) / kTile;
@@ -710,7 +729,8 @@ int launch_scan_op(const Op &op, int n, void *ws_ptr, size_t ws_bytes, hipStream
         }
         const int cap = static_cast<int>((ws_bytes - kScanHeaderBytes) / sizeof(uint64_t));
         hipLaunchKernelGGL((scan_lookback_kernel<BINARY, Op>), dim3(tiles), dim3(kBlock), 0, st, op,
-                           n, header, reinterpret_cast<uint64_t *>(after), tiles, cap);
+                           n, header, reinterpret_cast<uint64_t *>(after), tiles, cap, n_dev,
+                           n_min);
         HHS_LAUNCH_CHECK("scan_lookback_kernel");
         return HHS_OK;
     }
@@ -790,14 +810,15 @@ int launch_split(const Particles<T> &total, const Particles<T> &slow, const Part
 template <typename T>
 int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       const Particles<T> &fast, T pivot, int n, hhs_partition *part,
-                      unsigned long long *ticket, uint32_t seq, T h_slow, void *ws,
-                      size_t ws_bytes, hipStream_t st, double guard_band, double eta,
-                      double eps2) {
+                      unsigned long long *ticket, uint32_t seq, int *fast_count, T h_slow,
+                      void *ws, size_t ws_bytes, hipStream_t st, double guard_band, double eta,
+                      double eps2, const int *n_dev, int n_min) {
     HHS_REQUIRE(n > 0 && part && ticket, "split_hold: bad arguments");
+    HHS_REQUIRE(!n_dev || guard_band <= 0.0, "split_hold: a device-side count takes no guard");
     if constexpr (std::is_same<T, double>::value) {
         if (guard_band > 0.0) {
             SplitOp<T, true> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow,
-                                ticket, seq};
+                                ticket, seq, fast_count};
             op.guard = 1;
             op.gd.g = make_ts_guard(static_cast<double>(pivot), guard_band);
             op.gd.eta = eta;
@@ -806,8 +827,9 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
             return launch_scan_op<true>(op, n, ws, ws_bytes, st);
         }
     }
-    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, ticket, seq};
-    return launch_scan_op<true>(op, n, ws, ws_bytes, st);
+    SplitOp<T> op{total, slow, fast, nullptr, nullptr, pivot, n, part, 1, h_slow, ticket, seq,
+                  fast_count};
+    return launch_scan_op<true>(op, n, ws, ws_bytes, st, true, false, n_dev, n_min);
 }
 
 template <typename T>
@@ -822,12 +844,14 @@ int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, 
 
 template int launch_split_hold<double>(const Particles<double> &, const Particles<double> &,
                                        const Particles<double> &, double, int, hhs_partition *,
-                                       unsigned long long *, uint32_t, double, void *, size_t,
-                                       hipStream_t, double, double, double);
+                                       unsigned long long *, uint32_t, int *, double, void *,
+                                       size_t, hipStream_t, double, double, double, const int *,
+                                       int);
 template int launch_split_hold<float>(const Particles<float> &, const Particles<float> &,
                                       const Particles<float> &, float, int, hhs_partition *,
-                                      unsigned long long *, uint32_t, float, void *, size_t,
-                                      hipStream_t, double, double, double);
+                                      unsigned long long *, uint32_t, int *, float, void *,
+                                      size_t, hipStream_t, double, double, double, const int *,
+                                      int);
 template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
                                             int, double, hipStream_t);
 template int launch_lone_fast_fixup<float>(const Particles<float> &, const Particles<float> &,

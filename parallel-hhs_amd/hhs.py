@@ -13,6 +13,7 @@ hold_step / hold (src/hold.cuh:17-23,51-181).
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
 from dataclasses import dataclass
 
@@ -44,15 +45,27 @@ if not os.path.exists(LIB_PATH):
         "(or __graft_entry__.build()) first")
 
 # PyTorch-ROCm bundles its own HIP runtime with the same soname
-# (libamdhip64.so.7) as /opt/rocm's.  Loading torch first makes libhhs bind
-# to the runtime already in the process; loading libhhs first would bring in
-# a second HIP/HSA runtime next to torch's and crash on the first device call.
+# (libamdhip64.so.7) as /opt/rocm's.  Two runtimes in one process crash on
+# the first device call, so when PyTorch is installed its runtime file is
+# mapped first (RTLD_GLOBAL, without importing torch): libhhs's
+# libamdhip64.so.7 then resolves to it, and a later `import torch` finds the
+# same file already loaded.  Without PyTorch, libhhs's RUNPATH picks
+# /opt/rocm's.
+def _preload_torch_hip():
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+_preload_torch_hip()
+_lib = C.CDLL(LIB_PATH)
 try:
     import torch  # noqa: F401
 except ImportError:  # C++/ctypes-only use without PyTorch
     torch = None
-
-_lib = C.CDLL(LIB_PATH)
 
 
 class ParticlesF64(C.Structure):
@@ -126,6 +139,7 @@ _P64, _P32 = C.POINTER(ParticlesF64), C.POINTER(ParticlesF32)
 _SIGS = {
     "hhs_last_error": (C.c_char_p, []),
     "hhs_version": (C.c_char_p, []),
+    "hhs_runtime_check": (C.c_int, []),
     "hhs_device_count": (_i, [C.POINTER(_i)]),
     "hhs_malloc": (_i, [C.POINTER(_vp), _sz]),
     "hhs_free": (_i, [_vp]),
@@ -162,6 +176,7 @@ _SIGS = {
     "hhs_timesteps_sym_f64": (_i, [_P64, _i, _d, _d, _i, _vp, _sz, _vp]),
     "hhs_globals_set_subtree_max": (_i, [_vp, _i]),
     "hhs_globals_subtree_max": (_i, [_vp]),
+    "hhs_globals_set_speculation": (_i, [_vp, _i]),
     "hhs_globals_timestep_mode": (_i, [_vp]),
     "hhs_globals_kick_precision": (_i, [_vp]),
     "hhs_globals_root_steps": (C.c_longlong, [_vp]),
@@ -221,6 +236,11 @@ EXPORTED = tuple(_SIGS)
 
 def last_error() -> str:
     return _lib.hhs_last_error().decode()
+
+
+def runtime_check():
+    """Raises HHSError when two HIP runtimes are mapped into the process."""
+    check(_lib.hhs_runtime_check())
 
 
 def check(rc: int) -> None:
@@ -465,10 +485,12 @@ class Globals:
 
     def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48,
                  fast_timesteps: bool = False, kick_precision: int = -1,
-                 subtree_max: int | None = None):
+                 subtree_max: int | None = None, speculation: bool = True):
         """`kick_precision` is a KICK_* mode for this HOLD (-1: the process-wide
         setting).  kick_precision=KICK_F64_EXACT with exact timesteps is the
-        FP64-only HOLD mode, bit-exact with the FP64 CPU HOLD."""
+        FP64-only HOLD mode, bit-exact with the FP64 CPU HOLD.  `speculation`
+        queues each call's first fast recursion before its counts are read
+        back (same results, fewer idle gaps)."""
         h = C.c_void_p()
         check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
         self.h = h
@@ -478,6 +500,8 @@ class Globals:
             check(_lib.hhs_globals_set_kick_precision(h, kick_precision))
         if subtree_max is not None:
             check(_lib.hhs_globals_set_subtree_max(h, subtree_max))
+        if not speculation:
+            check(_lib.hhs_globals_set_speculation(h, 0))
 
     @property
     def subtree_max(self) -> int:

@@ -165,3 +165,26 @@ def test_timestep_autotune_cache_round_trip(hhs, tmp_path):
         hhs.autotune_clear()
     assert hhs.timesteps_get_config(3000, hhs.TIMESTEP_FP32) == base_fp32
     assert hhs.timesteps_get_config(3000, hhs.TIMESTEP_EXACT) == base_exact
+
+
+def test_one_hip_runtime_after_importing_torch(hhs):
+    """hhs.py maps PyTorch's bundled HIP runtime before libhhs, so the
+    process holds one runtime whichever of the two is imported first."""
+    code = ("import sys; sys.path.insert(0, %r); import hhs, torch; hhs.runtime_check();"
+            "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l};"
+            "assert len(maps) == 1, maps; print('ok')") % os.path.join(ROOT, "parallel-hhs_amd")
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr
+
+
+def test_runtime_check_reports_two_runtimes():
+    """Loading libhhs on its own (RUNPATH: /opt/rocm's runtime) and then
+    torch maps two HIP runtimes: hhs_runtime_check says so, naming both,
+    instead of the first device call crashing.  No device call is made."""
+    code = ("import ctypes; lib = ctypes.CDLL(%r); import torch;"
+            "lib.hhs_last_error.restype = ctypes.c_char_p;"
+            "rc = lib.hhs_runtime_check(); print(rc, lib.hhs_last_error().decode())") % LIB
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rc, msg = r.stdout.strip().split(" ", 1)
+    assert int(rc) == 3 and "two HIP runtimes" in msg and msg.count("libamdhip64") == 2

@@ -279,3 +279,36 @@ def test_hold_calc_timestep_flags_closed_form(hhs, torch_cuda, mode):
     st = g.stats()
     assert st["calls_per_level"] == [1, 2, 2] and st["ts_pairs"] == 10 and st["calls"] == 5
     assert g.simtime == dt
+
+
+@pytest.mark.parametrize("case", ["default", "no_subtree", "subtree512", "fp32_sets",
+                                  "fp64_exact", "tiny"])
+def test_hold_speculation_same_bits(hhs, torch_cuda, case):
+    """Speculative launches (each call's first fast recursion queued before
+    its counts are read back, with the set size read on the device) change
+    nothing: positions, velocities, statistics and simtime are the same bits
+    with speculation on and off, over several root steps, with and without
+    device subtrees, with subtree_max above the speculative subtree's limit,
+    on FP32 sets and in the FP64-only mode."""
+    import torch
+    n, kw, dtype = {
+        "default": (3000, dict(fast_timesteps=hhs.TIMESTEP_FP32), torch.float64),
+        "no_subtree": (1500, dict(fast_timesteps=hhs.TIMESTEP_FP32, subtree_max=0),
+                       torch.float64),
+        "subtree512": (1500, dict(subtree_max=512), torch.float64),
+        "fp32_sets": (1200, dict(), torch.float32),
+        "fp64_exact": (600, dict(kick_precision=hhs.KICK_F64_EXACT), torch.float64),
+        "tiny": (16, dict(subtree_max=0), torch.float64),
+    }[case]
+    eps, eta = 1e-3, 0.01
+    pos, vel, m = plummer(hhs, n, seed=7300 + n, eps2=eps * eps)
+    out = []
+    for spec in (True, False):
+        d = hhs.DeviceParticles.from_host(pos, vel, m, dtype=dtype)
+        g = hhs.Globals(n, eta, eps, max_level=40, speculation=spec, **kw)
+        g.hold(d, 0.125, 1.0 / 16)
+        torch_cuda.cuda.synchronize()
+        out.append((d.pos.cpu().numpy().tobytes(), d.vel.cpu().numpy().tobytes(), g.stats(),
+                    g.simtime))
+    assert out[0][2]["calls"] > 3
+    assert out[0] == out[1]
