@@ -196,8 +196,8 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
  * double-single FP32 copies of the positions and velocities, packed into
  * `ws`) is within 1e-6; HHS_TIMESTEP_FAST uses the one-sided kernel.  ws must
  * hold hhs_timesteps_sym_workspace_bytes(n) bytes: the larger of
- * ceil(n/1024) * n doubles (exact) and 64 n bytes + ceil(n/768) * n floats
- * (FP32).
+ * ceil(n/1024) * n doubles (exact) and 64 n bytes + (ceil(n/256) +
+ * ceil(n/768)) * n floats (FP32).
  * No reference counterpart (the reference evaluates every ordered pair). */
 size_t hhs_timesteps_sym_workspace_bytes(int n);
 int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,

@@ -121,11 +121,12 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
 // modes (FAST falls back to the one-sided kernel); ws holds
 // timesteps_sym_ws_bytes(n) bytes of workspace.  HOLD uses it from kSymMinN
 // particles up, where it beats the one-sided kernel (MI355X, FP32 on a
-// Salpeter-mass Plummer sphere, ms, symmetric vs one-sided: n = 16384 0.65
-// vs 0.37, 24576 0.96 vs 0.81, 32768 1.33 vs 1.36, 49152 2.45 vs 2.99, 65536
-// 3.00 vs 4.95, 98304 7.55 vs 11.4; EXACT: n = 32768 6.02 vs 6.21, 65536
+// Salpeter-mass Plummer sphere, ms, symmetric vs one-sided: n = 12288 0.24
+// vs 0.19, 16384 0.29 vs 0.31, 24576 0.57 vs 0.71, 32768 0.86 vs 1.23, 65536
+// 3.16 vs 5.00, 131072 10.9 vs 20.2; EXACT: n = 32768 6.02 vs 6.21, 65536
 // 17.1 vs 22.4, 131072 56.2 vs 92.4).
-constexpr int kSymMinN = 32768;
+constexpr int kSymMinN = 16384;       // FP32 mode
+constexpr int kSymMinNExact = 32768;  // exact mode
 size_t timesteps_sym_ws_bytes(int n);
 int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
                          double *ts, int n, double eta, double eps2, void *ws, size_t ws_bytes,

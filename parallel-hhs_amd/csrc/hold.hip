@@ -243,7 +243,8 @@ int alloc_subtree(hhs_globals *g, int n_max) {
 // kernel from kSymMinN particles when its workspace exists); FP32 sets in
 // FP32 with the formula's IEEE operations (the reference with T = float).
 int call_timesteps(hhs_globals *g, const Particles<double> &total, int n, hipStream_t st) {
-    if (g->ts_ws && n >= kSymMinN)
+    const int sym_min = g->timestep_mode == HHS_TIMESTEP_EXACT ? kSymMinNExact : kSymMinN;
+    if (g->ts_ws && n >= sym_min && timesteps_sym_ws_bytes(n) <= g->ts_ws_bytes)
         return launch_timesteps_sym(g->timestep_mode, total.pos, total.vel, total.m,
                                     total.timestep, n, g->eta, g->eps2, g->ts_ws, g->ts_ws_bytes,
                                     st);
@@ -521,10 +522,11 @@ int hhs_globals_create(hhs_globals **out, int n_total, double eta, double eps, i
         hhs_globals_destroy(g);
         return rc;
     }
-    // Symmetric timesteps from kSymMinN particles, when the partial minima
-    // (ceil(n/1024) * n doubles, 134 MB at n = 131072) stay within 2 GiB.
+    // Symmetric timesteps from kSymMinN particles, when the records and
+    // partial minima (358 MB at n = 131072, n^2 / 48 bytes) stay within
+    // 8 GiB (n_total up to ~640k).
     const size_t tsb = timesteps_sym_ws_bytes(n_total);
-    if (n_total >= kSymMinN && tsb <= (size_t{2} << 30)) {
+    if (n_total >= kSymMinN && tsb <= (size_t{8} << 30)) {
         if (hipMalloc(&g->ts_ws, tsb) != hipSuccess) {
             int rc = hip_status(hipErrorOutOfMemory, "globals_create", __FILE__, __LINE__);
             hhs_globals_destroy(g);

@@ -169,35 +169,47 @@ template <int S, bool SOFT> struct KickAcc<double, S, 3, SOFT> {
     }
 };
 
-// One workgroup: 64*S sinks (S per lane) against source chunk blockIdx.y of
-// K (split-K: small sink counts get more waves over the same sources), W
+// One workgroup: 64*S sinks (S per lane) against one source chunk of K
+// (split-K: small sink counts get more waves over the same sources), W
 // waves splitting the chunk (segment 0 then segment 1), per-wave totals
 // merged through LDS in wave order (deterministic).  With K == 1 the
 // epilogue writes acc <- a, vel += a * h; otherwise the chunk's FP64 partial
-// sums go to partial[k][i][3] for kick_reduce_kernel.  Tiles past the first
-// group's belong to the optional second group (sinks gb_*, segment 0 only);
-// its partials follow the first group's (i + n_sink).
+// sums go to partial[k][i][3] for kick_reduce_kernel.  Workgroups past the
+// first group's tiles_a * K_a belong to the optional second group (sinks
+// gb_*, segment 0 only) with its own chunk count K_b: in HOLD the two
+// groups have opposite shapes (few slow sinks over all sources, many fast
+// sinks over the few slow sources).  Each group's workgroups run chunk-major
+// (those in flight together share a chunk); the second group's partials
+// start after the first group's.
 template <typename T, int S, int W, int P, bool SOFT>
 __global__ __launch_bounds__(64 * W) void kick_kernel(
     const T *__restrict__ ga_pos, T *__restrict__ ga_vel, T *__restrict__ ga_acc, int n_sink,
     const T *__restrict__ gb_pos, T *__restrict__ gb_vel, T *__restrict__ gb_acc, int n_sink_b,
     const T *__restrict__ s0_pos, const T *__restrict__ s0_m, int n0,
     const T *__restrict__ s1_pos, const T *__restrict__ s1_m, int n1_a, T h, float eps2f,
-    double eps2, int K, double *__restrict__ partial, const float *__restrict__ mf) {
+    double eps2, int K_a, int K_b, double *__restrict__ partial, const float *__restrict__ mf) {
     __shared__ double red[(W > 1 ? W - 1 : 1) * 3 * S * 64];
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tiles_a = (n_sink + 64 * S - 1) / (64 * S);
-    const bool grp_b = static_cast<int>(blockIdx.x) >= tiles_a;
+    const int tiles_b = (n_sink_b + 64 * S - 1) / (64 * S);
+    const int wg_a = tiles_a * K_a;
+    const bool grp_b = static_cast<int>(blockIdx.x) >= wg_a;
+    const int bid = static_cast<int>(blockIdx.x) - (grp_b ? wg_a : 0);
+    const int ntile = grp_b ? tiles_b : tiles_a;
+    const int K = grp_b ? K_b : K_a;
+    const int chunk_id = bid / ntile;
     const T *si_pos = grp_b ? gb_pos : ga_pos;
     T *si_vel = grp_b ? gb_vel : ga_vel;
     T *si_acc = grp_b ? gb_acc : ga_acc;
     const int n1 = grp_b ? 0 : n1_a;
-    const int pofs = grp_b ? n_sink : 0;  // partial row of this group's first sink
-    const int n_rows = n_sink + n_sink_b;
+    // this group's partials: [K][rows of the group][3]
+    double *gpart = (partial && K > 1)
+                        ? partial + (grp_b ? 3 * static_cast<size_t>(K_a > 1 ? K_a : 0) * n_sink : 0)
+                        : nullptr;
     if (grp_b) n_sink = n_sink_b;
-    const int tile0 = (static_cast<int>(blockIdx.x) - (grp_b ? tiles_a : 0)) * (64 * S);
+    const int tile0 = (bid - chunk_id * ntile) * (64 * S);
 
     T xi[S], yi[S], zi[S];
 #pragma unroll
@@ -213,7 +225,7 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     // wave's contiguous share of it.
     const int total = n0 + n1;
     const int per = (total + K - 1) / K;
-    const int kb = min(static_cast<int>(blockIdx.y) * per, total);
+    const int kb = min(chunk_id * per, total);
     const int ke = min(kb + per, total);
     const int chunk = (ke - kb + W - 1) / W;
     const int jb = min(kb + wave * chunk, ke);
@@ -249,8 +261,8 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
             }
         }
     }
-    if (partial) {
-        double *q = partial + 3 * (static_cast<size_t>(blockIdx.y) * n_rows + pofs);
+    if (gpart) {
+        double *q = gpart + 3 * static_cast<size_t>(chunk_id) * n_sink;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int i = tile0 + s * 64 + lane;
@@ -303,29 +315,34 @@ __global__ __launch_bounds__(256) void masses_to_f32_kernel(const T *__restrict_
 }
 
 // Sum of the K chunk partials of each sink in chunk order (deterministic),
-// then the epilogue of kick_kernel; rows past n_sink are the second group's.
+// then the epilogue of kick_kernel, for the groups split over chunks (K_a,
+// K_b > 1; a group with one chunk was written by kick_kernel itself): rows
+// [0, ra) are the first group's, the next rb the second's.
 template <typename T>
 __global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ ga_vel,
                                                           T *__restrict__ ga_acc, int n_sink,
                                                           T *__restrict__ gb_vel,
                                                           T *__restrict__ gb_acc, int n_sink_b,
                                                           const double *__restrict__ partial,
-                                                          int K, T h) {
+                                                          int K_a, int K_b, T h) {
 #pragma clang fp contract(off)
     const int r = blockIdx.x * 256 + threadIdx.x;
-    const int n_rows = n_sink + n_sink_b;
-    if (r >= n_rows) return;
+    const int ra = K_a > 1 ? n_sink : 0, rb = K_b > 1 ? n_sink_b : 0;
+    if (r >= ra + rb) return;
+    const bool grp_b = r >= ra;
+    const int i = grp_b ? r - ra : r;
+    const int rows = grp_b ? n_sink_b : n_sink;
+    const int K = grp_b ? K_b : K_a;
+    const double *base = partial + (grp_b ? 3 * static_cast<size_t>(K_a > 1 ? K_a : 0) * n_sink : 0);
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int k = 0; k < K; ++k) {
-        const double *q = partial + 3 * (static_cast<size_t>(k) * n_rows + r);
+        const double *q = base + 3 * (static_cast<size_t>(k) * rows + i);
         ax += q[0];
         ay += q[1];
         az += q[2];
     }
-    const bool grp_b = r >= n_sink;
     T *si_vel = grp_b ? gb_vel : ga_vel;
     T *si_acc = grp_b ? gb_acc : ga_acc;
-    const int i = grp_b ? r - n_sink : r;
     const T a0 = static_cast<T>(ax), a1 = static_cast<T>(ay), a2 = static_cast<T>(az);
     if (si_acc) {
         si_acc[3 * i + 0] = a0;
@@ -337,35 +354,44 @@ __global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ ga_vel
     si_vel[3 * i + 2] = si_vel[3 * i + 2] + a2 * h;
 }
 
+// Chunk counts of the two sink groups (kick_kernel).
+struct SplitK {
+    int a, b;
+};
+
 template <typename T, int S, int W, int P>
-int launch_swp(const KickArgs<T> &a, int K, double *partial, const float *mf, hipStream_t st) {
+int launch_swp(const KickArgs<T> &a, SplitK K, double *partial, const float *mf,
+               hipStream_t st) {
     const int tile = 64 * S;
-    dim3 grid((a.n_sink + tile - 1) / tile + (a.n_sink_b + tile - 1) / tile, K), block(64 * W);
+    const int tiles_a = (a.n_sink + tile - 1) / tile, tiles_b = (a.n_sink_b + tile - 1) / tile;
+    dim3 grid(tiles_a * K.a + tiles_b * K.b), block(64 * W);
     const double eps2 = a.eps2;
     const float eps2f = static_cast<float>(a.eps2);
-    double *part = K > 1 ? partial : nullptr;
+    const bool split = K.a > 1 || K.b > 1;
+    double *part = split ? partial : nullptr;
     if (a.eps2 > 0.0)
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, true>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.sb_pos, a.sb_vel, a.sb_acc, a.n_sink_b,
-                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K,
-                           part, mf);
+                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K.a,
+                           K.b, part, mf);
     else
         hipLaunchKernelGGL((kick_kernel<T, S, W, P, false>), grid, block, 0, st, a.si_pos,
                            a.si_vel, a.si_acc, a.n_sink, a.sb_pos, a.sb_vel, a.sb_acc, a.n_sink_b,
-                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K,
-                           part, mf);
+                           a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K.a,
+                           K.b, part, mf);
     HHS_LAUNCH_CHECK("kick_kernel");
-    if (K > 1) {
-        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((a.n_sink + a.n_sink_b + 255) / 256),
-                           dim3(256), 0, st, a.si_vel, a.si_acc, a.n_sink, a.sb_vel, a.sb_acc,
-                           a.n_sink_b, part, K, a.h);
+    if (split) {
+        const int rows = (K.a > 1 ? a.n_sink : 0) + (K.b > 1 ? a.n_sink_b : 0);
+        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((rows + 255) / 256), dim3(256), 0, st,
+                           a.si_vel, a.si_acc, a.n_sink, a.sb_vel, a.sb_acc, a.n_sink_b, part,
+                           K.a, K.b, a.h);
         HHS_LAUNCH_CHECK("kick_reduce_kernel");
     }
     return HHS_OK;
 }
 
 template <typename T, int S, int W>
-int launch_sw(const KickArgs<T> &a, int P, int K, double *partial, const float *mf,
+int launch_sw(const KickArgs<T> &a, int P, SplitK K, double *partial, const float *mf,
               hipStream_t st) {
     if constexpr (std::is_same<T, double>::value) {
         if (P == 1) return launch_swp<T, S, W, 1>(a, K, partial, mf, st);
@@ -376,7 +402,7 @@ int launch_sw(const KickArgs<T> &a, int P, int K, double *partial, const float *
 }
 
 template <typename T, int S>
-int launch_s(const KickArgs<T> &a, int W, int P, int K, double *partial, const float *mf,
+int launch_s(const KickArgs<T> &a, int W, int P, SplitK K, double *partial, const float *mf,
              hipStream_t st) {
     switch (W) {
     case 1: return launch_sw<T, S, 1>(a, P, K, partial, mf, st);
@@ -437,7 +463,8 @@ constexpr int kSplitMinSources = 1024;
 int pick_split(int n_sink, int n_src, int S, int W) {
     const long long tiles = (n_sink + 64LL * S - 1) / (64LL * S);
     int K = 1;
-    if (n_src < kSplitMinSources) return K;  // the reduce launch would cost more than it saves
+    if (n_sink <= 0 || n_src < kSplitMinSources)  // the reduce launch would cost more
+        return K;
     while (K < 64 && tiles * W * K * 2 <= 8192 && n_src / (W * K * 2) >= 32 &&
            24ULL * n_sink * K * 2 <= kMaxPartialBytes)
         K *= 2;
@@ -467,14 +494,20 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
         c.sinks_per_lane = 1;
         c.waves = 1;
     }
-    // The exact mode keeps one sequential sum per sink.
-    const int K = (P == HHS_KICK_F64_EXACT) ? 1 : pick_split(n_rows, n_src_cfg,
-                                                             c.sinks_per_lane, c.waves);
+    // Each group's split from its own shape; the exact mode keeps one
+    // sequential sum per sink.
+    SplitK K{1, 1};
+    if (P != HHS_KICK_F64_EXACT) {
+        K.a = pick_split(a.n_sink, a.n0 + a.n1, c.sinks_per_lane, c.waves);
+        K.b = pick_split(a.n_sink_b, a.n0, c.sinks_per_lane, c.waves);
+    }
     static thread_local ThreadScratch scratch, mass_scratch;
     double *partial = nullptr;
-    if (K > 1) {
+    if (K.a > 1 || K.b > 1) {
+        const size_t rows_k = (K.a > 1 ? static_cast<size_t>(a.n_sink) * K.a : 0) +
+                              (K.b > 1 ? static_cast<size_t>(a.n_sink_b) * K.b : 0);
         void *p = nullptr;
-        int rc = scratch.acquire(24 * static_cast<size_t>(n_rows) * K, st, &p);
+        int rc = scratch.acquire(24 * rows_k, st, &p);
         if (rc) return rc;
         partial = static_cast<double *>(p);
     }
@@ -504,7 +537,7 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     case 4: rc = launch_s<T, 4>(a, c.waves, P, K, partial, mf, st); break;
     default: set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
     }
-    if (rc == HHS_OK && K > 1) rc = scratch.release(st);
+    if (rc == HHS_OK && (K.a > 1 || K.b > 1)) rc = scratch.release(st);
     if (rc == HHS_OK && conv) rc = mass_scratch.release(st);
     return rc;
 }

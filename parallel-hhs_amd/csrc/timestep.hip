@@ -253,30 +253,32 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
 namespace {
 // ---------------------------------------------------- symmetric evaluation
 // tau_ij = tau_ji bit for bit (pair_tau, pair_tau_ds), so each unordered
-// pair is evaluated once and feeds the minima of both particles.  Particles
-// are cut into blocks of 256 * S (S sinks per lane: 4 in the exact mode, 3
-// in FP32); workgroup (I, J), I <= J, takes sink block I (4 waves) against
-// source block J (one wave-uniform source at a time, scalar loads).  It keeps
+// pair is evaluated once and feeds the minima of both particles.  A
+// workgroup (4 waves) takes a block of sinks, S per lane, against a range
+// of sources (one wave-uniform source at a time, scalar loads).  It keeps
 // the row minima of its sinks in registers and reduces each source's column
-// minimum across the wave (shuffles) and the 4 waves (LDS).  Partial minima
-// go to P[K][i], the minimum of particle i over block K, written exactly
-// once:
+// minimum across the wave (shuffles) and the 4 waves (LDS).  Each partial
+// minimum is written exactly once and a second kernel takes the minimum of
+// a particle's partials.  No atomics; the minimum of a set does not depend
+// on the order, so the exact mode stays bit-exact.
+//
+// Exact mode: blocks of 1024 (4 sinks per lane); workgroup (I, J), I <= J,
+// takes sink block I against source block J, and P[K][i] is the minimum of
+// particle i over block K:
 //   (I, J), I < J: P[J][i] for i in I (rows), P[I][j] for j in J (columns);
 //   (I, I): P[I][i], all j != i of the block (rows only).
-// A second kernel takes tau_i = min_K P[K][i].  No atomics; the minimum of
-// a set does not depend on the order, so the exact mode stays bit-exact.
 //
-// The FP32 kernel reads double-single records (pair_math.h, packed once per
-// launch by pack_ds_kernel): its separations cost 3 FP32 operations instead
-// of an FP64 subtract and a conversion, and a source is one 64-byte scalar
-// load, fetched one iteration ahead.  With 3 sinks per lane (67 VGPRs, 7
-// waves per SIMD) it measured 11.2 ms at n = 131072 on MI355X against 14.0
-// ms for FP64 separations with 4 sinks per lane (5 waves); 3 sinks alone
-// gave 12.3 ms, 2 sinks 13.2 ms.
+// FP32 mode (timesteps_sym32_kernel, below): double-single records
+// (pair_math.h, packed once per launch by pack_ds_kernel), whose
+// separations cost 3 FP32 operations instead of an FP64 subtract and a
+// conversion; a source is one 64-byte scalar load, fetched one iteration
+// ahead.  With 3 sinks per lane (67 VGPRs, 7 waves per SIMD) and 768-source
+// blocks it measured 11.2 ms at n = 131072 on MI355X against 14.0 ms for
+// FP64 separations with 4 sinks per lane (5 waves).  The column minimum
+// stays on LDS permutes (__shfl_xor): the kernel is VALU-bound, and a DPP
+// reduction, VALU only, measured 12.2 ms against 10.9.
 constexpr int kSymSpl = 4;                      // exact mode: sinks per lane
 constexpr int kSymBlock = 64 * 4 * kSymSpl;     // 1024 particles per block
-constexpr int kSym32Spl = 3;                    // FP32 mode
-constexpr int kSym32Block = 64 * 4 * kSym32Spl; // 768
 
 template <typename V> __device__ __forceinline__ V vmin(V a, V b) { return b < a ? b : a; }
 
@@ -285,6 +287,7 @@ template <typename V> __device__ __forceinline__ V wave_min(V x) {
     for (int d = 32; d >= 1; d >>= 1) x = vmin(x, __shfl_xor(x, d, 64));
     return x;
 }
+
 
 __global__ __launch_bounds__(256) void timesteps_sym_kernel(const double *__restrict__ pos,
                                                             const double *__restrict__ vel,
@@ -382,70 +385,115 @@ __global__ __launch_bounds__(256) void pack_ds_kernel(const double *__restrict__
     for (int q = 0; q < 4; ++q) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
-template <bool SOFT>
+// FP32 symmetric kernel.  Particles are cut into units of 256 and sink
+// blocks of S units (256 S particles, S sinks per lane); workgroup w takes
+// sink block I against the 256 sources of unit u >= S I -- the triangle of
+// blocks against units, enumerated row by row (row I holds nu - S I
+// workgroups), so no workgroup is launched empty and the work comes in
+// units small enough to keep the end of the launch full.  Partial minima,
+// each written exactly once:
+//   u >= S (I + 1) (sources above the block): prow[u][i] for the block's
+//     sinks i, and pcol[I][j] for the unit's sources j;
+//   u in [S I, S I + S) (the block's own units): prow[u][i], j != i.
+// Particle i of block I, unit u_i, takes the minimum of prow[u][i] over
+// u in [S I, nu) and of pcol[K][i] over K < I.
+__device__ __forceinline__ long long sym_row_start(int I, int nu, int S) {
+    return static_cast<long long>(I) * nu - static_cast<long long>(S) * I * (I - 1) / 2;
+}
+template <int S>
+__device__ __forceinline__ void sym_tile(int w, int nu, int nb, int &I, int &u) {
+    const double h = nu + 0.5 * S;
+    int r = static_cast<int>((h - sqrt(h * h - 2.0 * S * static_cast<double>(w))) / S);
+    r = r < 0 ? 0 : (r >= nb ? nb - 1 : r);
+    while (r + 1 < nb && sym_row_start(r + 1, nu, S) <= w) ++r;
+    while (r > 0 && sym_row_start(r, nu, S) > w) --r;
+    I = r;
+    u = S * r + static_cast<int>(w - sym_row_start(r, nu, S));
+}
+
+template <int S, bool SOFT>
 __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__restrict__ rec, int n,
-                                                              float eta, float c_ff, float eps2,
-                                                              float *__restrict__ part) {
-    const int I = blockIdx.x, J = blockIdx.y;
-    if (I > J) return;
-    __shared__ float colmin[4][kSym32Block];
+                                                              int nu, int nb, float eta,
+                                                              float c_ff, float eps2,
+                                                              float *__restrict__ prow,
+                                                              float *__restrict__ pcol) {
+    int I, u;
+    sym_tile<S>(blockIdx.x, nu, nb, I, u);
+    I = __builtin_amdgcn_readfirstlane(I);
+    u = __builtin_amdgcn_readfirstlane(u);
+    __shared__ float colmin[4][256];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const float huge = TsConst<float>::huge;
-    float a[kSym32Spl][13], row[kSym32Spl];
-    int ii[kSym32Spl];
+    float a[S][13], row[S];
+    int ii[S];
 #pragma unroll
-    for (int s = 0; s < kSym32Spl; ++s) {
-        ii[s] = I * kSym32Block + s * 256 + wave * 64 + lane;
+    for (int s = 0; s < S; ++s) {
+        ii[s] = I * 256 * S + s * 256 + wave * 64 + lane;
         const int ic = ii[s] < n ? ii[s] : n - 1;
         const float *r = rec + kDsRec * static_cast<size_t>(ic);
 #pragma unroll
         for (int k = 0; k < 13; ++k) a[s][k] = r[k];
         row[s] = huge;
     }
-    const int j0 = J * kSym32Block;
-    const int j1 = min(j0 + kSym32Block, n);
-    if (I == J) {
+    const int j0 = u * 256;
+    const int j1 = min(j0 + 256, n);
+    if (u < S * (I + 1)) {
         for (int j = j0; j < j1; ++j) {
             const float *b = rec + kDsRec * static_cast<size_t>(j);
 #pragma unroll
-            for (int s = 0; s < kSym32Spl; ++s) {
+            for (int s = 0; s < S; ++s) {
                 const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
                 row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
             }
         }
     } else {
-        // the source record one iteration ahead (j1 > j0: block J is not empty)
-        float nb[13];
+        // the source record one iteration ahead (j1 > j0: unit u is not empty)
+        float nb_[13];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) nb[k] = rec[kDsRec * static_cast<size_t>(j0) + k];
+        for (int k = 0; k < 13; ++k) nb_[k] = rec[kDsRec * static_cast<size_t>(j0) + k];
         for (int j = j0; j < j1; ++j) {
             float b[13];
 #pragma unroll
-            for (int k = 0; k < 13; ++k) b[k] = nb[k];
+            for (int k = 0; k < 13; ++k) b[k] = nb_[k];
             const int jn = j + 1 < j1 ? j + 1 : j;
 #pragma unroll
-            for (int k = 0; k < 13; ++k) nb[k] = rec[kDsRec * static_cast<size_t>(jn) + k];
+            for (int k = 0; k < 13; ++k) nb_[k] = rec[kDsRec * static_cast<size_t>(jn) + k];
             float c = huge;
 #pragma unroll
-            for (int s = 0; s < kSym32Spl; ++s) {
+            for (int s = 0; s < S; ++s) {
                 const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
                 row[s] = vmin(row[s], v);
-                c = vmin(c, v);  // I < J: block I is full, every ii[s] < n
+                c = vmin(c, v);  // the block lies below unit u: every ii[s] < n
             }
             c = wave_min(c);
             if (lane == 0) colmin[wave][j - j0] = c;
         }
         __syncthreads();
-        for (int jj = threadIdx.x; jj < j1 - j0; jj += 256) {
+        const int jj = threadIdx.x;
+        if (jj < j1 - j0) {
             const float c =
                 vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
-            part[static_cast<size_t>(I) * n + j0 + jj] = c;  // P[I][j]
+            pcol[static_cast<size_t>(I) * n + j0 + jj] = c;
         }
     }
 #pragma unroll
-    for (int s = 0; s < kSym32Spl; ++s)
-        if (ii[s] < n) part[static_cast<size_t>(J) * n + ii[s]] = row[s];  // P[J][i]
+    for (int s = 0; s < S; ++s)
+        if (ii[s] < n) prow[static_cast<size_t>(u) * n + ii[s]] = row[s];
+}
+
+// tau_i from the FP32 kernel's partials (the layout above).
+template <int S>
+__global__ __launch_bounds__(256) void timesteps_sym32_reduce_kernel(
+    const float *__restrict__ prow, const float *__restrict__ pcol, int n, int nu,
+    double *__restrict__ ts) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int I = i / (256 * S);
+    float best = prow[static_cast<size_t>(S * I) * n + i];
+    for (int u = S * I + 1; u < nu; ++u) best = vmin(best, prow[static_cast<size_t>(u) * n + i]);
+    for (int K = 0; K < I; ++K) best = vmin(best, pcol[static_cast<size_t>(K) * n + i]);
+    ts[i] = static_cast<double>(best);
 }
 
 template <typename V>
@@ -459,14 +507,40 @@ __global__ __launch_bounds__(256) void timesteps_sym_reduce_kernel(const V *__re
     ts[i] = static_cast<double>(best);
 }
 
+// Sinks per lane of the FP32 kernel.  Three was fastest at every size
+// measured on MI355X (ms, S = 1 / 2 / 3: n = 16384 0.299 / 0.310 / 0.290,
+// 32768 0.942 / 0.901 / 0.856, 65536 3.36 / 3.20 / 3.16, 131072 12.25 /
+// 11.29 / 10.86): 256-source units keep enough workgroups at every size,
+// and larger blocks spend less per pair on the column minima.
+int sym32_spl(int) { return 3; }
+
 // Workspace of the FP32 kernel: the packed records (64 B per particle),
-// then the partial minima (ceil(n / 768) * n floats).
+// then prow (ceil(n / 256) * n floats) and pcol (one row per sink block,
+// ceil(n / (256 S)) * n floats; 358 MB in all at n = 131072).
 size_t sym32_rec_bytes(int n) {
     return (static_cast<size_t>(n) * kDsRec * sizeof(float) + 255) & ~static_cast<size_t>(255);
 }
 size_t sym32_ws_bytes(int n) {
-    const size_t nb = (static_cast<size_t>(n) + kSym32Block - 1) / kSym32Block;
-    return sym32_rec_bytes(n) + nb * static_cast<size_t>(n) * sizeof(float);
+    const size_t nu = (static_cast<size_t>(n) + 255) / 256;
+    const size_t S = static_cast<size_t>(sym32_spl(n));
+    const size_t nb = (static_cast<size_t>(n) + 256 * S - 1) / (256 * S);
+    return sym32_rec_bytes(n) + (nu + nb) * static_cast<size_t>(n) * sizeof(float);
+}
+
+template <int S>
+void launch_sym32(const float *rec, int n, int nu, float eta, float cff, float e2, float *prow,
+                  float *pcol, double *ts, hipStream_t s) {
+    const int nb = (n + 256 * S - 1) / (256 * S);
+    long long w = 0;
+    for (int I = 0; I < nb; ++I) w += nu - S * I;
+    if (e2 > 0.0f)
+        hipLaunchKernelGGL((timesteps_sym32_kernel<S, true>), dim3(static_cast<unsigned>(w)),
+                           dim3(256), 0, s, rec, n, nu, nb, eta, cff, e2, prow, pcol);
+    else
+        hipLaunchKernelGGL((timesteps_sym32_kernel<S, false>), dim3(static_cast<unsigned>(w)),
+                           dim3(256), 0, s, rec, n, nu, nb, eta, cff, e2, prow, pcol);
+    hipLaunchKernelGGL((timesteps_sym32_reduce_kernel<S>), dim3((n + 255) / 256), dim3(256), 0, s,
+                       prow, pcol, n, nu, ts);
 }
 
 }  // namespace
@@ -494,24 +568,22 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
                 timesteps_sym_ws_bytes(n));
     const dim3 red((n + 255) / 256);
     if (mode == HHS_TIMESTEP_FP32) {
-        const int nb = (n + kSym32Block - 1) / kSym32Block;
         float *rec = static_cast<float *>(ws);
-        float *part = reinterpret_cast<float *>(static_cast<char *>(ws) + sym32_rec_bytes(n));
+        const int nu = (n + 255) / 256;
+        float *prow = reinterpret_cast<float *>(static_cast<char *>(ws) + sym32_rec_bytes(n));
+        float *pcol = prow + static_cast<size_t>(nu) * n;
         hipLaunchKernelGGL(pack_ds_kernel, red, dim3(256), 0, s, pos, vel, m, n,
                            reinterpret_cast<float4 *>(rec));
         HHS_LAUNCH_CHECK("pack_ds_kernel");
         const float etaf = static_cast<float>(eta);
         const float cff = static_cast<float>(eta / TsConst<double>::sqrt2);
         const float e2 = static_cast<float>(eps2);
-        if (e2 > 0.0f)
-            hipLaunchKernelGGL(timesteps_sym32_kernel<true>, dim3(nb, nb), dim3(256), 0, s, rec, n,
-                               etaf, cff, e2, part);
-        else
-            hipLaunchKernelGGL(timesteps_sym32_kernel<false>, dim3(nb, nb), dim3(256), 0, s, rec,
-                               n, etaf, cff, e2, part);
+        switch (sym32_spl(n)) {
+        case 3: launch_sym32<3>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
+        case 2: launch_sym32<2>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
+        default: launch_sym32<1>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
+        }
         HHS_LAUNCH_CHECK("timesteps_sym32_kernel");
-        hipLaunchKernelGGL(timesteps_sym_reduce_kernel<float>, red, dim3(256), 0, s, part, nb, n,
-                           ts);
     } else {
         const int nb = (n + kSymBlock - 1) / kSymBlock;
         double *part = static_cast<double *>(ws);
