@@ -215,7 +215,7 @@ def hold_bench(hhs, torch, pos, vel, m, dt, label, eps=EPS, t_end=None, **modes)
             "dE_over_E": abs((ke1 + pe1 - ke0 - pe0) / (ke0 + pe0))}, d2.pos.cpu().numpy()
 
 
-def hold16k_bench(hhs, torch):
+def hold16k_bench(hhs, torch, fp64=True):
     """Plain HOLD of the N = 16384 equal-mass Plummer sphere (seed 2, eps =
     1e-4, eta = 0.01) to t = 0.125 in one root step (about ten power-of-two
     levels): the default mode (MIXED32 kicks, FP32 timesteps) beside the
@@ -225,10 +225,13 @@ def hold16k_bench(hhs, torch):
     pos, vel, m = hhs.ic_plummer(n, 2, 0, True, eps * eps)
     out, p_def = hold_bench(hhs, torch, pos, vel, m, t_end, "mixed kick, FP32 timesteps",
                             eps=eps, fast_timesteps=hhs.TIMESTEP_FP32)
+    out.update({"n": n, "eps": eps, "t_end": t_end})
+    if not fp64:
+        return out
     ref, p64 = hold_bench(hhs, torch, pos, vel, m, t_end, "FP64-only (exact kick and timesteps)",
                           eps=eps, kick_precision=hhs.KICK_F64_EXACT)
     ref["max_pos_diff_over_rms"] = float(np.abs(p_def - p64).max() / np.sqrt((p64 ** 2).mean()))
-    out.update({"n": n, "eps": eps, "t_end": t_end, "fp64_reference": ref,
+    out.update({"fp64_reference": ref,
                 "dE_ratio_to_fp64": out["dE_over_E"] / ref["dE_over_E"]})
     return out
 
@@ -561,7 +564,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-hold", action="store_true", help="skip the HOLD root-step timing")
+    ap.add_argument("--no-hold", action="store_true", help="skip both HOLD legs")
+    ap.add_argument("--no-root-step", action="store_true",
+                    help="skip the N=131072 HOLD root step")
+    ap.add_argument("--no-hold16k", action="store_true", help="skip the N=16384 plain HOLD")
+    ap.add_argument("--no-timesteps", action="store_true",
+                    help="skip the N=131072 timestep kernels")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in kick heuristic")
@@ -663,8 +671,9 @@ def main():
         "ic_seconds": round(t_ic, 2),
     }
     acc_gpu = d.acc.cpu().numpy()
-    out["timesteps"] = timesteps_bench(hhs, torch, d, stream)
-    log(f"timesteps: {out['timesteps']}")
+    if not args.no_timesteps:
+        out["timesteps"] = timesteps_bench(hhs, torch, d, stream)
+        log(f"timesteps: {out['timesteps']}")
     del d
     if not args.no_stream:
         out["stream_kernels"] = stream_bench(hhs, torch, stream)
@@ -674,7 +683,7 @@ def main():
     if not args.no_tune:
         out["timestep_autotune"] = tune_timesteps(hhs, torch)
         log(f"timestep autotune: {out['timestep_autotune']}")
-    if not args.no_hold:
+    if not args.no_hold and not args.no_root_step:
         out["hold_root_step"], p_mixed = hold_bench(
             hhs, torch, pos, vel, m, dt, "mixed kick, FP32 timesteps",
             fast_timesteps=hhs.TIMESTEP_FP32)
@@ -686,8 +695,8 @@ def main():
                                                  / np.sqrt((p64 ** 2).mean()))
             out["hold_root_step"]["fp64_reference"] = ref
             log(f"hold fp64: {ref}")
-    if not args.no_hold:
-        out["hold_n16384"] = hold16k_bench(hhs, torch)
+    if not args.no_hold and not args.no_hold16k:
+        out["hold_n16384"] = hold16k_bench(hhs, torch, fp64=not args.no_fp64_hold)
         log(f"hold n=16384: {out['hold_n16384']}")
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(hhs, torch, loop_sample=args.loop_sample,
