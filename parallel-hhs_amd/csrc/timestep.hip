@@ -597,9 +597,6 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
     return HHS_OK;
 }
 
-namespace {
-}  // namespace
-
 template int launch_timesteps<double>(const double *, const double *, const double *, double *,
                                       int, double, double, hipStream_t);
 template int launch_timesteps<float>(const float *, const float *, const float *, float *, int,
