@@ -1,7 +1,9 @@
-// Host-thread model of the ensemble kernel's work-sharing protocol
-// (parallel-hhs_amd/csrc/ensemble.hip, "work sharing"): the same slot
-// layout, claim word and steps, with std::atomic in place of the device's
-// agent-scope atomics and fences, threads in place of workgroups.  Test
+// Host-thread model of a work-sharing protocol for the ensemble kernel
+// (idle workgroups running 256-sink chunks of the pair passes of systems
+// still running; the device side is not in the library: its first GPU run
+// did not finish and was taken out, tests/test_gpu_ensemble_sharing.py):
+// the slot layout, claim word and steps, with std::atomic in place of the
+// device's agent-scope atomics and fences, threads in place of workgroups.  Test
 // infrastructure: built and run by tests/test_share_protocol.py, also under
 // ThreadSanitizer, whose happens-before checking covers the chunk outputs
 // (plain memory written by helpers, read by the owner).

@@ -1,7 +1,7 @@
-"""Host-thread model of the ensemble kernel's work-sharing protocol
-(tests/cpp/share_model.cpp; the device side is the "work sharing" part of
-parallel-hhs_amd/csrc/ensemble.hip).  CPU only: the model is built with g++,
-plain and under ThreadSanitizer, and must
+"""Host-thread model of a work-sharing protocol for the ensemble kernel
+(tests/cpp/share_model.cpp; its device side was taken out after its first
+GPU run did not finish, tests/test_gpu_ensemble_sharing.py).  CPU only: the
+model is built with g++, plain and under ThreadSanitizer, and must
 
 * give every pass's outputs exactly (each chunk run once, by the owner or a
   helper, with the descriptor of the pass it claimed) with no data race;
