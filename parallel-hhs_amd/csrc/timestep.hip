@@ -507,12 +507,20 @@ __global__ __launch_bounds__(256) void timesteps_sym_reduce_kernel(const V *__re
     ts[i] = static_cast<double>(best);
 }
 
-// Sinks per lane of the FP32 kernel.  Three was fastest at every size
-// measured on MI355X (ms, S = 1 / 2 / 3: n = 16384 0.299 / 0.310 / 0.290,
-// 32768 0.942 / 0.901 / 0.856, 65536 3.36 / 3.20 / 3.16, 131072 12.25 /
-// 11.29 / 10.86): 256-source units keep enough workgroups at every size,
-// and larger blocks spend less per pair on the column minima.
-int sym32_spl(int) { return 3; }
+// Sinks per lane of the FP32 kernel, measured on MI355X (ms, S = 1 / 2 /
+// 3: n = 16384 0.299 / 0.310 / 0.290, 32768 0.942 / 0.901 / 0.856, 65536
+// 3.36 / 3.20 / 3.16, 131072 12.25 / 11.29 / 10.86; S = 3 / 4: 16384 0.304
+// / 0.361, 32768 0.848 / 0.872, 65536 3.05 / 2.90, 131072 11.51 / 10.98 on
+// another build): four from 65536 particles, where larger blocks spend less
+// per pair on the column minima, three below, where they would leave too
+// few workgroups.  hhs_timesteps_sym_set_sinks_per_lane forces 3 or 4 (0:
+// by size); the workspace is sized for 3, which needs the most.
+std::atomic<int> g_sym32_spl{0};
+int sym32_spl(int n) {
+    const int v = g_sym32_spl.load();
+    if (v == 3 || v == 4) return v;
+    return n >= 65536 ? 4 : 3;
+}
 
 // Workspace of the FP32 kernel: the packed records (64 B per particle),
 // then prow (ceil(n / 256) * n floats) and pcol (one row per sink block,
@@ -522,7 +530,7 @@ size_t sym32_rec_bytes(int n) {
 }
 size_t sym32_ws_bytes(int n) {
     const size_t nu = (static_cast<size_t>(n) + 255) / 256;
-    const size_t S = static_cast<size_t>(sym32_spl(n));
+    const size_t S = 3;  // the largest workspace of the selectable sinks per lane
     const size_t nb = (static_cast<size_t>(n) + 256 * S - 1) / (256 * S);
     return sym32_rec_bytes(n) + (nu + nb) * static_cast<size_t>(n) * sizeof(float);
 }
@@ -579,6 +587,7 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
         const float cff = static_cast<float>(eta / TsConst<double>::sqrt2);
         const float e2 = static_cast<float>(eps2);
         switch (sym32_spl(n)) {
+        case 4: launch_sym32<4>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
         case 3: launch_sym32<3>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
         case 2: launch_sym32<2>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
         default: launch_sym32<1>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
@@ -633,6 +642,12 @@ int hhs_timesteps_mode_f64(const hhs_particles_f64 *sys, int n, double eta, doub
 }
 
 size_t hhs_timesteps_sym_workspace_bytes(int n) { return timesteps_sym_ws_bytes(n); }
+
+int hhs_timesteps_sym_set_sinks_per_lane(int s) {
+    HHS_REQUIRE(s == 0 || s == 3 || s == 4, "timesteps_sym: sinks per lane must be 0, 3 or 4");
+    g_sym32_spl.store(s);
+    return HHS_OK;
+}
 
 int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
                           void *ws, size_t ws_bytes, void *stream) {
