@@ -128,9 +128,16 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
 constexpr int kSymMinN = 16384;       // FP32 mode
 constexpr int kSymMinNExact = 32768;  // exact mode
 size_t timesteps_sym_ws_bytes(int n);
+// guard (FP32 mode, with guard_overflow): the decision guard's exact
+// recomputation of the timesteps within its band of a pivot runs as a stage
+// of this call on the whole GPU; *guard_overflow then names a device word
+// that is nonzero when the stage could not take them all, in which case the
+// split's own guard must run (launch_split_hold's guard_dev).
+struct TsGuard;
 int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
                          double *ts, int n, double eta, double eps2, void *ws, size_t ws_bytes,
-                         hipStream_t s);
+                         hipStream_t s, const TsGuard *guard = nullptr,
+                         const int **guard_overflow = nullptr);
 
 // ----------------------------------------------------------- scan / split
 // Workspace layout: u32 header[16] ([0] look-back epoch, [1] its completion
@@ -154,7 +161,9 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
 // slow set and reports the raw fast count, its lone-particle follow-up, and
 // the scatter-back of both sets with the second slow half-drift.
 // guard_band > 0 runs the decision guard of the approximate timestep modes
-// (pair_math.h) on the set's timesteps before the predicates are formed.
+// (pair_math.h) on the set's timesteps before the predicates are formed;
+// with guard_dev, only when that device word is nonzero (the timestep call
+// ran the guard itself, launch_timesteps_sym).
 // fast_count (optional device word) receives the reported fast count.
 // n_dev non-null: a speculative split whose count is read on the device
 // (n is then an upper bound, at most 2^18) and which runs only when that
@@ -165,7 +174,7 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       unsigned long long *ticket, uint32_t seq, int *fast_count, T h_slow,
                       void *ws, size_t ws_bytes, hipStream_t st, double guard_band = 0.0,
                       double eta = 0.0, double eps2 = 0.0, const int *n_dev = nullptr,
-                      int n_min = 0);
+                      int n_min = 0, const int *guard_dev = nullptr);
 template <typename T>
 int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
                            hipStream_t st);

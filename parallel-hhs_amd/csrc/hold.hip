@@ -242,16 +242,27 @@ int alloc_subtree(hhs_globals *g, int n_max) {
 // Timesteps of the call's set: FP64 sets in the globals' mode (symmetric
 // kernel from kSymMinN particles when its workspace exists); FP32 sets in
 // FP32 with the formula's IEEE operations (the reference with T = float).
-int call_timesteps(hhs_globals *g, const Particles<double> &total, int n, hipStream_t st) {
+// After the FP32 symmetric kernel the decision guard for pivots dt * 2^-k
+// runs as a stage of the call (launch_timesteps_sym); *guard_dev then names
+// the device word that tells the split whether it still has to guard.
+int call_timesteps(hhs_globals *g, const Particles<double> &total, int n, double dt,
+                   hipStream_t st, const int **guard_dev) {
+    *guard_dev = nullptr;
     const int sym_min = g->timestep_mode == HHS_TIMESTEP_EXACT ? kSymMinNExact : kSymMinN;
-    if (g->ts_ws && n >= sym_min && timesteps_sym_ws_bytes(n) <= g->ts_ws_bytes)
+    if (g->ts_ws && n >= sym_min && timesteps_sym_ws_bytes(n) <= g->ts_ws_bytes) {
+        const double band = guard_band(g);
+        const TsGuard gd = make_ts_guard(dt, band > 0.0 ? band : 0.0);
+        const bool stage = g->timestep_mode == HHS_TIMESTEP_FP32 && band > 0.0;
         return launch_timesteps_sym(g->timestep_mode, total.pos, total.vel, total.m,
                                     total.timestep, n, g->eta, g->eps2, g->ts_ws, g->ts_ws_bytes,
-                                    st);
+                                    st, stage ? &gd : nullptr, stage ? guard_dev : nullptr);
+    }
     return launch_timesteps_mode(g->timestep_mode, total.pos, total.vel, total.m, total.timestep,
                                  n, g->eta, g->eps2, st);
 }
-int call_timesteps(hhs_globals *g, const Particles<float> &total, int n, hipStream_t st) {
+int call_timesteps(hhs_globals *g, const Particles<float> &total, int n, float, hipStream_t st,
+                   const int **guard_dev) {
+    *guard_dev = nullptr;
     return launch_timesteps<float>(total.pos, total.vel, total.m, total.timestep, n,
                                    static_cast<float>(g->eta), static_cast<float>(g->eps2), st);
 }
@@ -372,8 +383,9 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
 
     int rc;
     // src/hold.cuh:74-76
+    const int *guard_dev = nullptr;
     if (calc) {
-        if ((rc = call_timesteps(g, total, n, st))) return rc;
+        if ((rc = call_timesteps(g, total, n, dt, st, &guard_dev))) return rc;
         S.ts_pairs += static_cast<long long>(n) * (n - 1);
     }
     if ((rc = ensure_level(g, clevel))) return rc;
@@ -391,7 +403,8 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
         seq = next_seq(g);
         rc = launch_split_hold<T>(total, slow, fast, dt, n, g->part_map + clevel,
                                   g->ticket_map + clevel, seq, g->next_n_dev + clevel + 1, h,
-                                  g->scan_ws, g->scan_ws_bytes, st, band, g->eta, g->eps2);
+                                  g->scan_ws, g->scan_ws_bytes, st, band, g->eta, g->eps2,
+                                  nullptr, 0, guard_dev);
         if (rc) return rc;
     }
     Queued child;

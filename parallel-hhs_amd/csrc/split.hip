@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void count_kernel(Op op, int n, int per_chu
     int c = 0;
     bool guarded = false;
     if constexpr (Op::kGuard) {
-        if (op.guard) {  // tiles of kItems per thread, the guard on each
+        if (op.guard_on()) {  // tiles of kItems per thread, the guard on each
             guarded = true;
             for (int base = b; base < e_end; base += kTile) {
                 double ts[kItems];
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void scan_chunks_kernel(Op op, int n, int p
         // a single chunk has no count pass: the guard runs on the loaded
         // records (the emitted timesteps are the refined ones)
         if constexpr (Op::kGuard) {
-            if (op.guard && gridDim.x == 1) {
+            if (gridDim.x == 1 && op.guard_on()) {
                 double ts[kItems];
 #pragma unroll
                 for (int k = 0; k < kItems; ++k) ts[k] = base + k * kBlock + t < c_end ? it[k].ts : 0.0;
@@ -322,7 +322,7 @@ __device__ int tile_value_sum(const Op &op, int n, int k, int *s_red) {
     const int t = threadIdx.x;
     int c = 0;
     if constexpr (Op::kGuard) {
-        if (op.guard) {
+        if (op.guard_on()) {
             double ts[kItems];
 #pragma unroll
             for (int q = 0; q < kItems; ++q) {
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(Op op, int n, uin
         if (e < c_end) it[k] = op.load(e);
     }
     if constexpr (Op::kGuard) {
-        if (op.guard) {  // the guard on the loaded records (uniform branch)
+        if (op.guard_on()) {  // the guard on the loaded records (uniform branch)
             double ts[kItems];
 #pragma unroll
             for (int k = 0; k < kItems; ++k) ts[k] = base + k * kBlock + t < c_end ? it[k].ts : 0.0;
@@ -575,9 +575,15 @@ template <typename T, bool GUARD = false> struct SplitOp {
     unsigned long long *ticket;
     uint32_t seq;
     int *fast_count;  // device word: the reported fast count, the next level's n
-    // HOLD only (GUARD): the decision guard, when guard != 0
+    // HOLD only (GUARD): the decision guard, when guard != 0 and, with
+    // guard_dev, *guard_dev != 0 (the timestep call ran it already unless
+    // its stage overflowed; the word is read once per workgroup)
     int guard;
     SplitGuard gd;
+    const int *guard_dev;
+    __device__ bool guard_on() const {
+        return guard && (!guard_dev || __builtin_amdgcn_readfirstlane(*guard_dev) != 0);
+    }
     using Item = SplitRec<T>;
     // src/split.cuh:115 -- NaN compares false and counts as slow.
     __device__ int value_at(int e) const { return total.timestep[e] <= pivot ? 1 : 0; }
@@ -812,7 +818,7 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
                       const Particles<T> &fast, T pivot, int n, hhs_partition *part,
                       unsigned long long *ticket, uint32_t seq, int *fast_count, T h_slow,
                       void *ws, size_t ws_bytes, hipStream_t st, double guard_band, double eta,
-                      double eps2, const int *n_dev, int n_min) {
+                      double eps2, const int *n_dev, int n_min, const int *guard_dev) {
     HHS_REQUIRE(n > 0 && part && ticket, "split_hold: bad arguments");
     HHS_REQUIRE(!n_dev || guard_band <= 0.0, "split_hold: a device-side count takes no guard");
     if constexpr (std::is_same<T, double>::value) {
@@ -824,6 +830,7 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
             op.gd.eta = eta;
             op.gd.c_ff = eta / TsConst<double>::sqrt2;
             op.gd.eps2 = eps2;
+            op.guard_dev = guard_dev;
             return launch_scan_op<true>(op, n, ws, ws_bytes, st);
         }
     }
@@ -846,12 +853,12 @@ template int launch_split_hold<double>(const Particles<double> &, const Particle
                                        const Particles<double> &, double, int, hhs_partition *,
                                        unsigned long long *, uint32_t, int *, double, void *,
                                        size_t, hipStream_t, double, double, double, const int *,
-                                       int);
+                                       int, const int *);
 template int launch_split_hold<float>(const Particles<float> &, const Particles<float> &,
                                       const Particles<float> &, float, int, hhs_partition *,
                                       unsigned long long *, uint32_t, int *, float, void *,
                                       size_t, hipStream_t, double, double, double, const int *,
-                                      int);
+                                      int, const int *);
 template int launch_lone_fast_fixup<double>(const Particles<double> &, const Particles<double> &,
                                             int, double, hipStream_t);
 template int launch_lone_fast_fixup<float>(const Particles<float> &, const Particles<float> &,

@@ -362,11 +362,16 @@ __global__ __launch_bounds__(256) void timesteps_sym_kernel(const double *__rest
 }
 
 // Double-single records of particles [0, n) (pair_math.h, kDsRec floats each).
+// With a guard fix stage it also clears the stage's counters (gfix_hdr).
 __global__ __launch_bounds__(256) void pack_ds_kernel(const double *__restrict__ pos,
                                                       const double *__restrict__ vel,
                                                       const double *__restrict__ m, int n,
-                                                      float4 *__restrict__ rec) {
+                                                      float4 *__restrict__ rec, int *gfix_hdr) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0 && gfix_hdr) {
+        gfix_hdr[0] = 0;  // flagged timesteps
+        gfix_hdr[1] = 0;  // more than kGuardFixCap of them
+    }
     if (i >= n) return;
     float v[kDsRec];
 #pragma unroll
@@ -482,18 +487,96 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
         if (ii[s] < n) prow[static_cast<size_t>(u) * n + ii[s]] = row[s];
 }
 
-// tau_i from the FP32 kernel's partials (the layout above).
+// ------------------------------------------------ decision guard stage
+// The HOLD split's decision guard (pair_math.h) recomputes with the exact
+// formula every timestep within its band of a pivot, each over all n
+// sources on one workgroup of the split -- at n = 65536 ... 131072 about
+// 190 us per call, on the look-back's critical path.  After the FP32
+// symmetric kernel the flagged timesteps are instead listed by the reduce
+// kernel and recomputed by the whole GPU: guard_partial_kernel gives each
+// workgroup 256 sources and every flagged particle one exact pair per
+// thread (workgroup minimum per particle), guard_final_kernel takes the
+// minimum over the workgroups and writes it into ts.  A minimum does not
+// depend on order, so the values are the guard's bit for bit.  More than
+// kGuardFixCap flagged timesteps set the overflow word, and the split then
+// runs its own guard as before (on every flagged timestep; the fixed ones
+// give the same values again).
+constexpr int kGuardFixCap = 256;
+
+struct GuardFix {
+    int on;
+    TsGuard g;
+    double eta, c_ff, eps2;
+    int *hdr;          // [0] flagged count, [1] overflow
+    int *list;         // flagged particles (n entries)
+    double *partial;   // [kGuardFixCap][groups]
+};
+
+// tau_i from the FP32 kernel's partials (the layout above); with the guard
+// stage, the timesteps within the band of a pivot are listed.
 template <int S>
 __global__ __launch_bounds__(256) void timesteps_sym32_reduce_kernel(
     const float *__restrict__ prow, const float *__restrict__ pcol, int n, int nu,
-    double *__restrict__ ts) {
+    double *__restrict__ ts, GuardFix gf) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int I = i / (256 * S);
     float best = prow[static_cast<size_t>(S * I) * n + i];
     for (int u = S * I + 1; u < nu; ++u) best = vmin(best, prow[static_cast<size_t>(u) * n + i]);
     for (int K = 0; K < I; ++K) best = vmin(best, pcol[static_cast<size_t>(K) * n + i]);
-    ts[i] = static_cast<double>(best);
+    const double v = static_cast<double>(best);
+    ts[i] = v;
+    if (gf.on && guard_near_pivot(gf.g, v)) {
+        const int k = atomicAdd(gf.hdr, 1);
+        if (k < n) gf.list[k] = i;
+    }
+}
+
+// Workgroup g: sources [256 g, 256 g + 256) against every listed particle.
+__global__ __launch_bounds__(256) void guard_partial_kernel(const double *__restrict__ pos,
+                                                            const double *__restrict__ vel,
+                                                            const double *__restrict__ m, int n,
+                                                            GuardFix gf) {
+    __shared__ double s_min[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int cnt = min(__builtin_amdgcn_readfirstlane(gf.hdr[0]), kGuardFixCap);
+    const int j = blockIdx.x * 256 + t;
+    for (int f = 0; f < cnt; ++f) {
+        const int s = __builtin_amdgcn_readfirstlane(gf.list[f]);
+        double best = TsConst<double>::huge;
+        if (j < n)
+            pair_timestep<double>(pos[3 * s], pos[3 * s + 1], pos[3 * s + 2], vel[3 * s],
+                                  vel[3 * s + 1], vel[3 * s + 2], m[s], pos[3 * j],
+                                  pos[3 * j + 1], pos[3 * j + 2], vel[3 * j], vel[3 * j + 1],
+                                  vel[3 * j + 2], m[j], gf.eta, gf.c_ff, gf.eps2, j != s, best);
+        best = wave_min(best);
+        if (lane == 0) s_min[wave] = best;
+        __syncthreads();
+        if (t == 0)
+            gf.partial[static_cast<size_t>(f) * gridDim.x + blockIdx.x] =
+                vmin(vmin(s_min[0], s_min[1]), vmin(s_min[2], s_min[3]));
+        __syncthreads();
+    }
+}
+
+// One workgroup: the listed particles' exact timesteps into ts.
+__global__ __launch_bounds__(256) void guard_final_kernel(double *__restrict__ ts, int groups,
+                                                          GuardFix gf) {
+    __shared__ double s_min[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int total = __builtin_amdgcn_readfirstlane(gf.hdr[0]);
+    const int cnt = min(total, kGuardFixCap);
+    for (int f = 0; f < cnt; ++f) {
+        double best = TsConst<double>::huge;
+        for (int c = t; c < groups; c += 256)
+            best = vmin(best, gf.partial[static_cast<size_t>(f) * groups + c]);
+        best = wave_min(best);
+        if (lane == 0) s_min[wave] = best;
+        __syncthreads();
+        if (t == 0) ts[gf.list[f]] = vmin(vmin(s_min[0], s_min[1]), vmin(s_min[2], s_min[3]));
+        __syncthreads();
+    }
+    if (t == 0) gf.hdr[1] = total > kGuardFixCap ? 1 : 0;
 }
 
 template <typename V>
@@ -528,16 +611,24 @@ int sym32_spl(int n) {
 size_t sym32_rec_bytes(int n) {
     return (static_cast<size_t>(n) * kDsRec * sizeof(float) + 255) & ~static_cast<size_t>(255);
 }
-size_t sym32_ws_bytes(int n) {
+size_t sym32_core_bytes(int n) {
     const size_t nu = (static_cast<size_t>(n) + 255) / 256;
     const size_t S = 3;  // the largest workspace of the selectable sinks per lane
     const size_t nb = (static_cast<size_t>(n) + 256 * S - 1) / (256 * S);
-    return sym32_rec_bytes(n) + (nu + nb) * static_cast<size_t>(n) * sizeof(float);
+    return sym32_rec_bytes(n) + (((nu + nb) * static_cast<size_t>(n) * sizeof(float) + 255) &
+                                 ~static_cast<size_t>(255));
 }
+// the guard stage: 256-byte header, the list (n ints), the partial minima
+size_t sym32_guard_bytes(int n) {
+    const size_t groups = (static_cast<size_t>(n) + 255) / 256;
+    return 256 + ((static_cast<size_t>(n) * sizeof(int) + 255) & ~static_cast<size_t>(255)) +
+           static_cast<size_t>(kGuardFixCap) * groups * sizeof(double);
+}
+size_t sym32_ws_bytes(int n) { return sym32_core_bytes(n) + sym32_guard_bytes(n); }
 
 template <int S>
 void launch_sym32(const float *rec, int n, int nu, float eta, float cff, float e2, float *prow,
-                  float *pcol, double *ts, hipStream_t s) {
+                  float *pcol, double *ts, const GuardFix &gf, hipStream_t s) {
     const int nb = (n + 256 * S - 1) / (256 * S);
     long long w = 0;
     for (int I = 0; I < nb; ++I) w += nu - S * I;
@@ -548,7 +639,7 @@ void launch_sym32(const float *rec, int n, int nu, float eta, float cff, float e
         hipLaunchKernelGGL((timesteps_sym32_kernel<S, false>), dim3(static_cast<unsigned>(w)),
                            dim3(256), 0, s, rec, n, nu, nb, eta, cff, e2, prow, pcol);
     hipLaunchKernelGGL((timesteps_sym32_reduce_kernel<S>), dim3((n + 255) / 256), dim3(256), 0, s,
-                       prow, pcol, n, nu, ts);
+                       prow, pcol, n, nu, ts, gf);
 }
 
 }  // namespace
@@ -563,7 +654,8 @@ size_t timesteps_sym_ws_bytes(int n) {
 
 int launch_timesteps_sym(int mode, const double *pos, const double *vel, const double *m,
                          double *ts, int n, double eta, double eps2, void *ws, size_t ws_bytes,
-                         hipStream_t s) {
+                         hipStream_t s, const TsGuard *guard, const int **guard_overflow) {
+    if (guard_overflow) *guard_overflow = nullptr;
     HHS_REQUIRE(n >= 0, "timesteps: negative n");
     if (n == 0) return HHS_OK;
     if (mode == HHS_TIMESTEP_FAST)
@@ -580,19 +672,39 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
         const int nu = (n + 255) / 256;
         float *prow = reinterpret_cast<float *>(static_cast<char *>(ws) + sym32_rec_bytes(n));
         float *pcol = prow + static_cast<size_t>(nu) * n;
+        // the guard stage's area follows the kernel's partial minima
+        char *gbase = static_cast<char *>(ws) + sym32_core_bytes(n);
+        GuardFix gf{};
+        if (guard && guard_overflow) {
+            gf.on = 1;
+            gf.g = *guard;
+            gf.eta = eta;
+            gf.c_ff = eta / TsConst<double>::sqrt2;
+            gf.eps2 = eps2;
+            gf.hdr = reinterpret_cast<int *>(gbase);
+            gf.list = reinterpret_cast<int *>(gbase + 256);
+            gf.partial = reinterpret_cast<double *>(
+                gbase + 256 + ((static_cast<size_t>(n) * sizeof(int) + 255) & ~size_t{255}));
+        }
         hipLaunchKernelGGL(pack_ds_kernel, red, dim3(256), 0, s, pos, vel, m, n,
-                           reinterpret_cast<float4 *>(rec));
+                           reinterpret_cast<float4 *>(rec), gf.on ? gf.hdr : nullptr);
         HHS_LAUNCH_CHECK("pack_ds_kernel");
         const float etaf = static_cast<float>(eta);
         const float cff = static_cast<float>(eta / TsConst<double>::sqrt2);
         const float e2 = static_cast<float>(eps2);
         switch (sym32_spl(n)) {
-        case 4: launch_sym32<4>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
-        case 3: launch_sym32<3>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
-        case 2: launch_sym32<2>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
-        default: launch_sym32<1>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, s); break;
+        case 4: launch_sym32<4>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, gf, s); break;
+        default: launch_sym32<3>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, gf, s); break;
         }
         HHS_LAUNCH_CHECK("timesteps_sym32_kernel");
+        if (gf.on) {
+            hipLaunchKernelGGL(guard_partial_kernel, red, dim3(256), 0, s, pos, vel, m, n, gf);
+            HHS_LAUNCH_CHECK("guard_partial_kernel");
+            hipLaunchKernelGGL(guard_final_kernel, dim3(1), dim3(256), 0, s, ts,
+                               static_cast<int>(red.x), gf);
+            HHS_LAUNCH_CHECK("guard_final_kernel");
+            *guard_overflow = gf.hdr + 1;
+        }
     } else {
         const int nb = (n + kSymBlock - 1) / kSymBlock;
         double *part = static_cast<double *>(ws);

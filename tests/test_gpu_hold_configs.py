@@ -213,3 +213,35 @@ def test_hold_n16384_default_matches_fp64_run(hhs, torch_cuda):
 
 def test_hold_n131072_root_step_default_matches_fp64_run(hhs, torch_cuda):
     _config_vs_fp64(hhs, torch_cuda, 131072, 3, 1, 1.0 / 64)
+
+
+def test_guard_stage_overflow_on_a_lattice(hhs, torch_cuda):
+    """The symmetric kernel's guard stage takes at most 256 flagged
+    timesteps; beyond that the split runs its own guard.  A cubic lattice at
+    rest gives every particle the same free-fall timestep; with the root
+    step just above it, all 16384 timesteps lie in the band (the stage
+    overflows), and the FP32 run must still make the exact run's decisions:
+    the same bits."""
+    nx, ny, nz, a = 16, 32, 32, 1.0 / 32
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    pos = (g.reshape(-1, 3) * a).astype(np.float64)
+    n = len(pos)
+    vel = np.zeros_like(pos)
+    m = np.full(n, 1.0 / n)
+    eps, eta = 1e-4, 0.01
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    hhs.timesteps(d, eta=eta, eps2=eps * eps, fast=hhs.TIMESTEP_EXACT)
+    torch_cuda.cuda.synchronize()
+    t64 = d.timestep.cpu().numpy()
+    assert np.all(t64 == t64[0])
+    dt = float(t64[0]) * (1 + 2e-6)
+    out = []
+    for mode in (hhs.TIMESTEP_FP32, hhs.TIMESTEP_EXACT):
+        dd = hhs.DeviceParticles.from_host(pos, vel, m)
+        gg = hhs.Globals(n, eta, eps, fast_timesteps=mode)
+        gg.hold_step(dd, dt)
+        out.append((dd, gg))
+    torch_cuda.cuda.synchronize()
+    (d0, g0), (d1, g1) = out
+    print("lattice stats", g0.stats()["calls"], g1.stats()["calls"])
+    _assert_same_bits(out)
