@@ -297,8 +297,8 @@ __global__ __launch_bounds__(64 * W) void kick_kernel(
     }
 }
 
-// Source masses of both segments, rounded once to FP32 (segment 1 after
-// segment 0).
+// Interaction count from which MIXED32 kicks read FP32 masses converted once
+// per launch instead of converting each source mass in every tile.
 constexpr double kConvMinWork = 536870912.0; // 2^29 interactions
 
 // Source masses of both segments rounded once to FP32 (segment 1 after
