@@ -200,11 +200,11 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
  * ceil(n/768)) * n floats (FP32).
  * No reference counterpart (the reference evaluates every ordered pair). */
 size_t hhs_timesteps_sym_workspace_bytes(int n);
-/* Sinks per lane of the FP32 symmetric kernel: 0 (default) picks by size (4
- * from 65536 particles, 3 below); 3 or 4 forces one; the workspace size
- * above does not change.  Process-wide; results
- * stay within the FP32 bar either way (the minimum over a particle's pairs
- * does not depend on the blocking).  No reference counterpart. */
+/* Sinks per lane of the FP32 symmetric kernel: 0 (default: 3, the faster
+ * at every size measured) or 3 or 4; the workspace size above does not
+ * change.  Process-wide; the timesteps are the same bits either way (the
+ * minimum over a particle's pairs does not depend on the blocking).  No
+ * reference counterpart. */
 int hhs_timesteps_sym_set_sinks_per_lane(int s);
 int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
                           void *ws, size_t ws_bytes, void *stream);

@@ -169,19 +169,21 @@ __device__ __forceinline__ float ds_sep(float hi, float lo, float hj, float lj) 
     return (hi - hj) + (lo - lj);
 }
 
-// FP32 form of the pair's own timestep min(tau_ff, tau_fb) on double-single
+// FP32 form of the pair's two candidates tau_ff and tau_fb on double-single
 // records (a: the sink in registers, b: the source), for the symmetric
 // kernel: the arithmetic of pair_timestep_f32 from the separations on, each
 // candidate divided by (1 - dtau/2) through the hardware reciprocal (~1 ulp)
-// instead of being folded as a fraction.  NaN candidates are ignored and
-// the result starts at 1e30.  Symmetric in (i, j) bit for bit: a swap flips
-// the sign of every separation exactly (the operations on hi and lo parts
-// are negated operand for operand), and the formula sees them only through
-// r^2, v^2, v.r and m_i + m_j.  SOFT: eps2 > 0 in FP32, so r^2 > 0 and the
-// r = 0 case needs no test.
+// instead of being folded as a fraction.  Either candidate may be NaN; the
+// caller folds them into running minima that start at 1e30 with fminf
+// (fold_tau_ds), which ignores NaN.  Every candidate is >= +0 or NaN (c_ff,
+// eta > 0; the denominators lie in [1/2, inf]).  Symmetric in (i, j) bit
+// for bit: a swap flips the sign of every separation exactly (the operations
+// on hi and lo parts are negated operand for operand), and the formula sees
+// them only through r^2, v^2, v.r and m_i + m_j.  SOFT: eps2 > 0 in FP32, so
+// r^2 > 0 and the r = 0 case needs no test.
 template <bool SOFT>
-__device__ __forceinline__ float pair_tau_ds(const float (&a)[13], const float *b, float eta,
-                                             float c_ff, float eps2) {
+__device__ __forceinline__ void pair_tau_ds(const float (&a)[13], const float *b, float eta,
+                                            float c_ff, float eps2, float &t_ff, float &t_fb) {
     const float drx = ds_sep(a[0], a[1], b[0], b[1]), dry = ds_sep(a[2], a[3], b[2], b[3]),
                 drz = ds_sep(a[4], a[5], b[4], b[5]);
     const float dvx = ds_sep(a[6], a[7], b[6], b[7]), dvy = ds_sep(a[8], a[9], b[8], b[9]),
@@ -197,12 +199,19 @@ __device__ __forceinline__ float pair_tau_ds(const float (&a)[13], const float *
     // fminf(NaN, 1) = 1, as the (d < 1) ? d : 1 cap
     const float tff = c_ff * __builtin_amdgcn_rsqf(mu * rinv * rinv2);
     float d = __builtin_fminf(1.5f * tff * u, 1.0f);
-    const float t_ff = tff * __builtin_amdgcn_rcpf(__builtin_fmaf(-0.5f, d, 1.0f));
+    t_ff = tff * __builtin_amdgcn_rcpf(__builtin_fmaf(-0.5f, d, 1.0f));
     const float vinv = __builtin_amdgcn_rsqf(dv2);
     const float tfb = eta * r * vinv;
     d = __builtin_fminf(tfb * u * __builtin_fmaf(mu * (vinv * vinv), rinv, 1.0f), 1.0f);
-    const float t_fb = tfb * __builtin_amdgcn_rcpf(__builtin_fmaf(-0.5f, d, 1.0f));
-    return __builtin_fminf(__builtin_fminf(t_ff, t_fb), TsConst<float>::huge);
+    t_fb = tfb * __builtin_amdgcn_rcpf(__builtin_fmaf(-0.5f, d, 1.0f));
+}
+
+// acc <- min(acc, t_ff, t_fb), NaN candidates ignored (acc is never NaN: it
+// starts at 1e30).  One v_min3_f32.  Gives the bits of folding the pair
+// minimum clamped to 1e30, min(acc, min(min(t_ff, t_fb), 1e30)), since
+// acc <= 1e30.
+__device__ __forceinline__ float fold_tau_ds(float acc, float t_ff, float t_fb) {
+    return __builtin_fminf(__builtin_fminf(acc, t_ff), t_fb);
 }
 
 // ------------------------------------------------------ decision guard

@@ -288,6 +288,17 @@ template <typename V> __device__ __forceinline__ V wave_min(V x) {
     return x;
 }
 
+// Minimum over the wave of values in [+0, 1e30]: for non-negative floats the
+// order of the bit patterns as unsigned integers is the order of the values,
+// so each step is one v_min_u32 (a float minimum of the permuted value would
+// first have to quiet a possible signalling NaN).
+__device__ __forceinline__ float wave_min_nonneg(float x) {
+    unsigned u = __float_as_uint(x);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) u = min(u, static_cast<unsigned>(__shfl_xor(static_cast<int>(u), d, 64)));
+    return __uint_as_float(u);
+}
+
 
 __global__ __launch_bounds__(256) void timesteps_sym_kernel(const double *__restrict__ pos,
                                                             const double *__restrict__ vel,
@@ -448,8 +459,9 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
             const float *b = rec + kDsRec * static_cast<size_t>(j);
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
-                row[s] = (j != ii[s]) ? vmin(row[s], v) : row[s];
+                float t_ff, t_fb;
+                pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2, t_ff, t_fb);
+                row[s] = (j != ii[s]) ? fold_tau_ds(row[s], t_ff, t_fb) : row[s];
             }
         }
     } else {
@@ -467,11 +479,12 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
             float c = huge;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const float v = pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2);
-                row[s] = vmin(row[s], v);
-                c = vmin(c, v);  // the block lies below unit u: every ii[s] < n
+                float t_ff, t_fb;
+                pair_tau_ds<SOFT>(a[s], b, eta, c_ff, eps2, t_ff, t_fb);
+                row[s] = fold_tau_ds(row[s], t_ff, t_fb);
+                c = fold_tau_ds(c, t_ff, t_fb);  // the block lies below unit u: every ii[s] < n
             }
-            c = wave_min(c);
+            c = wave_min_nonneg(c);
             if (lane == 0) colmin[wave][j - j0] = c;
         }
         __syncthreads();
@@ -592,17 +605,18 @@ __global__ __launch_bounds__(256) void timesteps_sym_reduce_kernel(const V *__re
 
 // Sinks per lane of the FP32 kernel, measured on MI355X (ms, S = 1 / 2 /
 // 3: n = 16384 0.299 / 0.310 / 0.290, 32768 0.942 / 0.901 / 0.856, 65536
-// 3.36 / 3.20 / 3.16, 131072 12.25 / 11.29 / 10.86; S = 3 / 4: 16384 0.304
-// / 0.361, 32768 0.848 / 0.872, 65536 3.05 / 2.90, 131072 11.51 / 10.98 on
-// another build): four from 65536 particles, where larger blocks spend less
-// per pair on the column minima, three below, where they would leave too
-// few workgroups.  hhs_timesteps_sym_set_sinks_per_lane forces 3 or 4 (0:
-// by size); the workspace is sized for 3, which needs the most.
+// 3.36 / 3.20 / 3.16, 131072 12.25 / 11.29 / 10.86).  With compare-and-select
+// minima four sinks per lane won from 65536 particles (131072: 10.97 against
+// 10.87), since larger blocks spend less per pair on the column minima.
+// With the minima as v_min3_f32 folds and v_min_u32 wave steps, three win
+// at every size (S = 3 / 4: 16384 0.276 / 0.339, 32768 0.749 / 0.837, 65536
+// 2.77 / 2.81, 131072 10.25 / 10.57; same bits).
+// hhs_timesteps_sym_set_sinks_per_lane forces 3 or 4 (0: the default, 3);
+// the workspace is sized for 3, which needs the most.
 std::atomic<int> g_sym32_spl{0};
-int sym32_spl(int n) {
+int sym32_spl(int) {
     const int v = g_sym32_spl.load();
-    if (v == 3 || v == 4) return v;
-    return n >= 65536 ? 4 : 3;
+    return v == 4 ? 4 : 3;
 }
 
 // Workspace of the FP32 kernel: the packed records (64 B per particle),
