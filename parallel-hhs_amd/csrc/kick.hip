@@ -456,9 +456,12 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
 // and chunk) stay within kMaxPartialBytes.  At N = 4096 a plain launch is
 // 512 waves, each running 512 sources back to back with nothing to hide
 // its latency.  MI355X, square kicks: N = 4096 48 -> 21 us, 8192 96 ->
-// 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.
+// 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.  From 512 sources up
+// (1024 before): the HOLD calls of a few hundred to a thousand particles
+// gain more than the reduce launch costs (N = 1024 to t = 1 0.375 ->
+// 0.361 s, N = 16384 in one root step 0.415 -> 0.402 s; 768 fell between).
 constexpr size_t kMaxPartialBytes = size_t{4} << 20;
-constexpr int kSplitMinSources = 1024;
+constexpr int kSplitMinSources = 512;
 
 int pick_split(int n_sink, int n_src, int S, int W) {
     const long long tiles = (n_sink + 64LL * S - 1) / (64LL * S);
