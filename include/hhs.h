@@ -196,8 +196,9 @@ int hhs_timesteps_f64(const hhs_particles_f64 *sys, int n, double eta, double ep
  * double-single FP32 copies of the positions and velocities, packed into
  * `ws`) is within 1e-6; HHS_TIMESTEP_FAST uses the one-sided kernel.  ws must
  * hold hhs_timesteps_sym_workspace_bytes(n) bytes: the larger of
- * ceil(n/1024) * n doubles (exact) and 64 n bytes + (ceil(n/256) +
- * ceil(n/768)) * n floats (FP32).
+ * ceil(n/1024) * n doubles (exact) and, for FP32, 64 n bytes + the larger of
+ * (ceil(n/256) + ceil(n/768)) * n and (ceil(k/64) + ceil(k/768)) * k floats
+ * with k = min(n, 65536), + the decision guard's area.
  * No reference counterpart (the reference evaluates every ordered pair). */
 size_t hhs_timesteps_sym_workspace_bytes(int n);
 /* Sinks per lane of the FP32 symmetric kernel: 0 (default: 3, the faster
@@ -206,6 +207,12 @@ size_t hhs_timesteps_sym_workspace_bytes(int n);
  * minimum over a particle's pairs does not depend on the blocking).  No
  * reference counterpart. */
 int hhs_timesteps_sym_set_sinks_per_lane(int s);
+/* Source unit of the FP32 symmetric kernel: 0 (default: 64 sources up to
+ * 65536 particles, 256 above) or 64, 128 or 256 (64 and 128 only up to
+ * 65536 particles; larger counts keep 256).  Process-wide; the workspace
+ * size above covers every choice, and the timesteps are the same bits
+ * whatever the unit.  No reference counterpart. */
+int hhs_timesteps_sym_set_unit(int u);
 int hhs_timesteps_sym_f64(const hhs_particles_f64 *sys, int n, double eta, double eps2, int mode,
                           void *ws, size_t ws_bytes, void *stream);
 int hhs_timesteps_f32(const hhs_particles_f32 *sys, int n, float eta, float eps2,

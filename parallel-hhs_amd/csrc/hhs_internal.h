@@ -120,12 +120,12 @@ int launch_timesteps_mode(int mode, const double *pos, const double *vel, const 
 // Symmetric evaluation (each unordered pair once) for the EXACT and FP32
 // modes (FAST falls back to the one-sided kernel); ws holds
 // timesteps_sym_ws_bytes(n) bytes of workspace.  HOLD uses it from kSymMinN
-// particles up, where it beats the one-sided kernel (MI355X, FP32 on a
-// Salpeter-mass Plummer sphere, ms, symmetric vs one-sided: n = 12288 0.24
-// vs 0.19, 16384 0.29 vs 0.31, 24576 0.57 vs 0.71, 32768 0.86 vs 1.23, 65536
-// 3.16 vs 5.00, 131072 10.9 vs 20.2; EXACT: n = 32768 6.02 vs 6.21, 65536
-// 17.1 vs 22.4, 131072 56.2 vs 92.4).
-constexpr int kSymMinN = 16384;       // FP32 mode
+// particles up, where it beats the one-sided kernel (MI355X, FP32 on an
+// equal-mass Plummer sphere, us, symmetric with 64-source units vs
+// one-sided: n = 6144 62 vs 62, 8192 76 vs 101, 12288 141 vs 215, 16384 215
+// vs 358, 32768 688 vs 1263, 131072 10240 vs 19443; EXACT: n = 32768 6.02
+// vs 6.21 ms, 65536 17.1 vs 22.4, 131072 56.2 vs 92.4).
+constexpr int kSymMinN = 8192;        // FP32 mode
 constexpr int kSymMinNExact = 32768;  // exact mode
 size_t timesteps_sym_ws_bytes(int n);
 // guard (FP32 mode, with guard_overflow): the decision guard's exact

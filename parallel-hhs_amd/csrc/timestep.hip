@@ -14,6 +14,7 @@
 // The reference's early `return` before a barrier (src/timestep.cuh:70 vs
 // :89,:95) has no counterpart: there are no barriers here, and lanes past
 // n compute on a clamped sink and do not store.
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <type_traits>
@@ -401,18 +402,20 @@ __global__ __launch_bounds__(256) void pack_ds_kernel(const double *__restrict__
     for (int q = 0; q < 4; ++q) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
-// FP32 symmetric kernel.  Particles are cut into units of 256 and sink
-// blocks of S units (256 S particles, S sinks per lane); workgroup w takes
-// sink block I against the 256 sources of unit u >= S I -- the triangle of
-// blocks against units, enumerated row by row (row I holds nu - S I
-// workgroups), so no workgroup is launched empty and the work comes in
-// units small enough to keep the end of the launch full.  Partial minima,
-// each written exactly once:
-//   u >= S (I + 1) (sources above the block): prow[u][i] for the block's
+// FP32 symmetric kernel.  Particles are cut into units of U sources and
+// sink blocks of 256 S particles (S sinks per lane), R = 256 S / U units
+// each; workgroup w takes sink block I against the U sources of unit
+// u >= R I -- the triangle of blocks against units, enumerated row by row
+// (row I holds nu - R I workgroups), so no workgroup is launched empty and
+// the work comes in units small enough to keep the end of the launch full.
+// Partial minima, each written exactly once:
+//   u >= R (I + 1) (sources above the block): prow[u][i] for the block's
 //     sinks i, and pcol[I][j] for the unit's sources j;
-//   u in [S I, S I + S) (the block's own units): prow[u][i], j != i.
-// Particle i of block I, unit u_i, takes the minimum of prow[u][i] over
-// u in [S I, nu) and of pcol[K][i] over K < I.
+//   u in [R I, R I + R) (the block's own units): prow[u][i], j != i.
+// Particle i of block I takes the minimum of prow[u][i] over u in [R I, nu)
+// and of pcol[K][i] over K < I.  Each unordered pair's two timesteps are the
+// same bits whichever particle is the sink (the separations only change
+// sign), and a minimum does not depend on order, so U changes no bit.
 __device__ __forceinline__ long long sym_row_start(int I, int nu, int S) {
     return static_cast<long long>(I) * nu - static_cast<long long>(S) * I * (I - 1) / 2;
 }
@@ -427,17 +430,19 @@ __device__ __forceinline__ void sym_tile(int w, int nu, int nb, int &I, int &u) 
     u = S * r + static_cast<int>(w - sym_row_start(r, nu, S));
 }
 
-template <int S, bool SOFT>
+template <int S, int U, bool SOFT>
 __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__restrict__ rec, int n,
                                                               int nu, int nb, float eta,
                                                               float c_ff, float eps2,
                                                               float *__restrict__ prow,
                                                               float *__restrict__ pcol) {
+    constexpr int R = 256 * S / U;  // units of U sources per sink block
+    static_assert(R * U == 256 * S, "a sink block holds whole units");
     int I, u;
-    sym_tile<S>(blockIdx.x, nu, nb, I, u);
+    sym_tile<R>(blockIdx.x, nu, nb, I, u);
     I = __builtin_amdgcn_readfirstlane(I);
     u = __builtin_amdgcn_readfirstlane(u);
-    __shared__ float colmin[4][256];
+    __shared__ float colmin[4][U];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const float huge = TsConst<float>::huge;
@@ -452,9 +457,9 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
         for (int k = 0; k < 13; ++k) a[s][k] = r[k];
         row[s] = huge;
     }
-    const int j0 = u * 256;
-    const int j1 = min(j0 + 256, n);
-    if (u < S * (I + 1)) {
+    const int j0 = u * U;
+    const int j1 = min(j0 + U, n);
+    if (u < R * (I + 1)) {
         for (int j = j0; j < j1; ++j) {
             const float *b = rec + kDsRec * static_cast<size_t>(j);
 #pragma unroll
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
         }
         __syncthreads();
         const int jj = threadIdx.x;
-        if (jj < j1 - j0) {
+        if (jj < U && jj < j1 - j0) {
             const float c =
                 vmin(vmin(colmin[0][jj], colmin[1][jj]), vmin(colmin[2][jj], colmin[3][jj]));
             pcol[static_cast<size_t>(I) * n + j0 + jj] = c;
@@ -527,16 +532,32 @@ struct GuardFix {
 
 // tau_i from the FP32 kernel's partials (the layout above); with the guard
 // stage, the timesteps within the band of a pivot are listed.
-template <int S>
-__global__ __launch_bounds__(256) void timesteps_sym32_reduce_kernel(
+// 64 particles per workgroup, one per lane; its kSymRedSlices waves take
+// every kSymRedSlices-th row of the partials (a particle has up to n / U of
+// them, too many for one thread's serial loop) and are combined in LDS.
+// The result is the minimum over the same values in any grouping.
+constexpr int kSymRedSlices = 16;
+template <int S, int U>
+__global__ __launch_bounds__(64 * kSymRedSlices) void timesteps_sym32_reduce_kernel(
     const float *__restrict__ prow, const float *__restrict__ pcol, int n, int nu,
     double *__restrict__ ts, GuardFix gf) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int I = i / (256 * S);
-    float best = prow[static_cast<size_t>(S * I) * n + i];
-    for (int u = S * I + 1; u < nu; ++u) best = vmin(best, prow[static_cast<size_t>(u) * n + i]);
-    for (int K = 0; K < I; ++K) best = vmin(best, pcol[static_cast<size_t>(K) * n + i]);
+    constexpr int R = 256 * S / U;
+    __shared__ float part[kSymRedSlices][64];
+    const int lane = threadIdx.x & 63;
+    const int slice = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = blockIdx.x * 64 + lane;
+    const int ic = i < n ? i : n - 1;
+    const int I = ic / (256 * S);
+    float best = TsConst<float>::huge;
+    for (int u = R * I + slice; u < nu; u += kSymRedSlices)
+        best = vmin(best, prow[static_cast<size_t>(u) * n + ic]);
+    for (int K = slice; K < I; K += kSymRedSlices)
+        best = vmin(best, pcol[static_cast<size_t>(K) * n + ic]);
+    part[slice][lane] = best;
+    __syncthreads();
+    if (slice != 0 || i >= n) return;
+#pragma unroll
+    for (int q = 1; q < kSymRedSlices; ++q) best = vmin(best, part[q][lane]);
     const double v = static_cast<double>(best);
     ts[i] = v;
     if (gf.on && guard_near_pivot(gf.g, v)) {
@@ -619,18 +640,42 @@ int sym32_spl(int) {
     return v == 4 ? 4 : 3;
 }
 
+// Source units of the FP32 kernel.  Below ~32768 particles 256-source units
+// leave too few workgroups, each a long serial pass (n = 8192: 132
+// workgroups); units of 64 sources give 4x as many, shorter ones, for the
+// same bits.  MI355X, kernel and reduce, us, U = 64 / 128 / 256 (one-sided
+// FP32 kernel): n = 6144 62 / 112 / 205 (62), 8192 76 / 115 / 206 (101),
+// 12288 141 / 163 / 220 (215), 16384 215 / 220 / 254 (358), 32768 688 /
+// 698 / 729, 65536 2628 / 2643 / 2685, 131072 10362 / 10282 / 10240.
+// Units narrower than 256 are used up to kSymNarrowMaxN particles (their row
+// partials grow as n^2 / U).  hhs_timesteps_sym_set_unit forces 64, 128 or
+// 256 (0: by size).
+constexpr int kSymNarrowMaxN = 65536;
+std::atomic<int> g_sym32_unit{0};
+int sym32_unit(int n) {
+    const int f = g_sym32_unit.load();
+    if (f == 256 || ((f == 64 || f == 128) && n <= kSymNarrowMaxN)) return f;
+    return n <= kSymNarrowMaxN ? 64 : 256;
+}
+
 // Workspace of the FP32 kernel: the packed records (64 B per particle),
-// then prow (ceil(n / 256) * n floats) and pcol (one row per sink block,
-// ceil(n / (256 S)) * n floats; 358 MB in all at n = 131072).
+// then prow (ceil(n / U) * n floats) and pcol (one row per sink block,
+// ceil(n / (256 S)) * n floats; 358 MB in all at n = 131072).  Sized for
+// the narrowest unit any count up to n may use, so that a workspace sized
+// for n serves every smaller count.
 size_t sym32_rec_bytes(int n) {
     return (static_cast<size_t>(n) * kDsRec * sizeof(float) + 255) & ~static_cast<size_t>(255);
 }
-size_t sym32_core_bytes(int n) {
-    const size_t nu = (static_cast<size_t>(n) + 255) / 256;
+size_t sym32_region_bytes(size_t n, size_t U) {
     const size_t S = 3;  // the largest workspace of the selectable sinks per lane
-    const size_t nb = (static_cast<size_t>(n) + 256 * S - 1) / (256 * S);
-    return sym32_rec_bytes(n) + (((nu + nb) * static_cast<size_t>(n) * sizeof(float) + 255) &
-                                 ~static_cast<size_t>(255));
+    const size_t nu = (n + U - 1) / U, nb = (n + 256 * S - 1) / (256 * S);
+    return ((nu + nb) * n * sizeof(float) + 255) & ~static_cast<size_t>(255);
+}
+size_t sym32_core_bytes(int n) {
+    const size_t nn = static_cast<size_t>(n);
+    const size_t narrow = sym32_region_bytes(std::min<size_t>(nn, kSymNarrowMaxN), 64);
+    const size_t wide = sym32_region_bytes(nn, 256);
+    return sym32_rec_bytes(n) + std::max(narrow, wide);
 }
 // the guard stage: 256-byte header, the list (n ints), the partial minima
 size_t sym32_guard_bytes(int n) {
@@ -640,20 +685,32 @@ size_t sym32_guard_bytes(int n) {
 }
 size_t sym32_ws_bytes(int n) { return sym32_core_bytes(n) + sym32_guard_bytes(n); }
 
-template <int S>
-void launch_sym32(const float *rec, int n, int nu, float eta, float cff, float e2, float *prow,
-                  float *pcol, double *ts, const GuardFix &gf, hipStream_t s) {
+template <int S, int U>
+void launch_sym32(const float *rec, int n, float eta, float cff, float e2, float *prow,
+                  double *ts, const GuardFix &gf, hipStream_t s) {
+    constexpr int R = 256 * S / U;
+    const int nu = (n + U - 1) / U;
+    float *pcol = prow + static_cast<size_t>(nu) * n;
     const int nb = (n + 256 * S - 1) / (256 * S);
     long long w = 0;
-    for (int I = 0; I < nb; ++I) w += nu - S * I;
+    for (int I = 0; I < nb; ++I) w += nu - R * I;
     if (e2 > 0.0f)
-        hipLaunchKernelGGL((timesteps_sym32_kernel<S, true>), dim3(static_cast<unsigned>(w)),
+        hipLaunchKernelGGL((timesteps_sym32_kernel<S, U, true>), dim3(static_cast<unsigned>(w)),
                            dim3(256), 0, s, rec, n, nu, nb, eta, cff, e2, prow, pcol);
     else
-        hipLaunchKernelGGL((timesteps_sym32_kernel<S, false>), dim3(static_cast<unsigned>(w)),
+        hipLaunchKernelGGL((timesteps_sym32_kernel<S, U, false>), dim3(static_cast<unsigned>(w)),
                            dim3(256), 0, s, rec, n, nu, nb, eta, cff, e2, prow, pcol);
-    hipLaunchKernelGGL((timesteps_sym32_reduce_kernel<S>), dim3((n + 255) / 256), dim3(256), 0, s,
-                       prow, pcol, n, nu, ts, gf);
+    hipLaunchKernelGGL((timesteps_sym32_reduce_kernel<S, U>), dim3((n + 63) / 64),
+                       dim3(64 * kSymRedSlices), 0, s, prow, pcol, n, nu, ts, gf);
+}
+template <int S>
+void launch_sym32_u(int U, const float *rec, int n, float eta, float cff, float e2, float *prow,
+                    double *ts, const GuardFix &gf, hipStream_t s) {
+    switch (U) {
+    case 64: launch_sym32<S, 64>(rec, n, eta, cff, e2, prow, ts, gf, s); break;
+    case 128: launch_sym32<S, 128>(rec, n, eta, cff, e2, prow, ts, gf, s); break;
+    default: launch_sym32<S, 256>(rec, n, eta, cff, e2, prow, ts, gf, s); break;
+    }
 }
 
 }  // namespace
@@ -683,9 +740,7 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
     const dim3 red((n + 255) / 256);
     if (mode == HHS_TIMESTEP_FP32) {
         float *rec = static_cast<float *>(ws);
-        const int nu = (n + 255) / 256;
         float *prow = reinterpret_cast<float *>(static_cast<char *>(ws) + sym32_rec_bytes(n));
-        float *pcol = prow + static_cast<size_t>(nu) * n;
         // the guard stage's area follows the kernel's partial minima
         char *gbase = static_cast<char *>(ws) + sym32_core_bytes(n);
         GuardFix gf{};
@@ -706,9 +761,10 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
         const float etaf = static_cast<float>(eta);
         const float cff = static_cast<float>(eta / TsConst<double>::sqrt2);
         const float e2 = static_cast<float>(eps2);
+        const int U = sym32_unit(n);
         switch (sym32_spl(n)) {
-        case 4: launch_sym32<4>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, gf, s); break;
-        default: launch_sym32<3>(rec, n, nu, etaf, cff, e2, prow, pcol, ts, gf, s); break;
+        case 4: launch_sym32_u<4>(U, rec, n, etaf, cff, e2, prow, ts, gf, s); break;
+        default: launch_sym32_u<3>(U, rec, n, etaf, cff, e2, prow, ts, gf, s); break;
         }
         HHS_LAUNCH_CHECK("timesteps_sym32_kernel");
         if (gf.on) {
@@ -768,6 +824,13 @@ int hhs_timesteps_mode_f64(const hhs_particles_f64 *sys, int n, double eta, doub
 }
 
 size_t hhs_timesteps_sym_workspace_bytes(int n) { return timesteps_sym_ws_bytes(n); }
+
+int hhs_timesteps_sym_set_unit(int u) {
+    HHS_REQUIRE(u == 0 || u == 64 || u == 128 || u == 256,
+                "timesteps_sym: unit must be 0, 64, 128 or 256");
+    g_sym32_unit.store(u);
+    return HHS_OK;
+}
 
 int hhs_timesteps_sym_set_sinks_per_lane(int s) {
     HHS_REQUIRE(s == 0 || s == 3 || s == 4, "timesteps_sym: sinks per lane must be 0, 3 or 4");

@@ -175,6 +175,7 @@ _SIGS = {
     "hhs_timesteps_sym_workspace_bytes": (_sz, [_i]),
     "hhs_timesteps_sym_f64": (_i, [_P64, _i, _d, _d, _i, _vp, _sz, _vp]),
     "hhs_timesteps_sym_set_sinks_per_lane": (_i, [_i]),
+    "hhs_timesteps_sym_set_unit": (_i, [_i]),
     "hhs_globals_set_subtree_max": (_i, [_vp, _i]),
     "hhs_globals_subtree_max": (_i, [_vp]),
     "hhs_globals_set_speculation": (_i, [_vp, _i]),

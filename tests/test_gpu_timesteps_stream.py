@@ -182,6 +182,36 @@ def test_timesteps_symmetric_large(hhs, oracle, torch_cuda, n):
     assert np.max(np.abs(tc / ta - 1.0)) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 257, 1000, 4097, 9000, 16384, 20000])
+def test_timesteps_sym_fp32_units_same_bits(hhs, torch_cuda, n):
+    """The FP32 symmetric kernel's source unit (64, 128 or 256 sources per
+    workgroup tile) and sinks per lane (3 or 4) change no bit of the
+    timesteps: every unordered pair gives the same two values whichever
+    particle is the sink, and a minimum does not depend on the grouping.
+    Ragged sizes, a unit narrower than a wave, and the default choice."""
+    eps2 = 1e-8
+    pos, vel, m = plummer(hhs, n, seed=720 + n, mass_model=1, eps2=eps2)
+    if n > 10:
+        pos[9] = pos[4] + 1e-7   # a close pair
+    d = hhs.DeviceParticles.from_host(pos, vel, m)
+    out = {}
+    try:
+        for unit in (0, 64, 128, 256):
+            for spl in (3, 4):
+                hhs.check(hhs._lib.hhs_timesteps_sym_set_unit(unit))
+                hhs.check(hhs._lib.hhs_timesteps_sym_set_sinks_per_lane(spl))
+                d.timestep.zero_()
+                hhs.timesteps_sym(d, eta=0.01, eps2=eps2, mode=hhs.TIMESTEP_FP32)
+                torch_cuda.cuda.synchronize()
+                out[(unit, spl)] = d.timestep.cpu().numpy().tobytes()
+    finally:
+        hhs._lib.hhs_timesteps_sym_set_unit(0)
+        hhs._lib.hhs_timesteps_sym_set_sinks_per_lane(0)
+    ref = out[(256, 3)]
+    assert all(v == ref for v in out.values())
+    assert hhs._lib.hhs_timesteps_sym_set_unit(96) == hhs.HHS_ERR_INVALID
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_timestep_autotune_installs_a_valid_config(hhs, oracle, torch_cuda, tmp_path, mode):
     """The timestep autotuner installs a (W, K) for the size class and mode,
