@@ -362,6 +362,12 @@ int hhs_globals_subtree_max(const hhs_globals *g);
  * out not to be the one the recursion makes does nothing.  Results are the
  * same bits either way.  No reference counterpart. */
 int hhs_globals_set_speculation(hhs_globals *g, int on);
+/* Kick partials applied by their readers (default on): a call whose second
+ * fast recursion runs as a device subtree (or that has no fast set) leaves
+ * its split-K kick's partial sums to that subtree (fast sinks) and to its
+ * scatter-back (slow sinks) instead of a reduce launch.  Results are the
+ * same bits either way.  No reference counterpart. */
+int hhs_globals_set_fused_reduce(hhs_globals *g, int on);
 int hhs_globals_timestep_mode(const hhs_globals *g);
 int hhs_globals_kick_precision(const hhs_globals *g);
 /* Root steps completed by hhs_evolve_f64 (restored from a snapshot on

@@ -1397,7 +1397,7 @@ static_assert(kSubLds <= kSmallSrc, "an LDS subtree's calls take the quarter-spl
 template <bool SOFT, bool LDS, bool GUARD>
 __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void subtree_kernel(
     EnsArgs a, Particles<double> total, int n, int clevel, double dt, int calc, SubtreeAcc *acc,
-    const int *n_dev) {
+    const int *n_dev, KickPending kp) {
     constexpr int TEAM = LDS ? kSubTeam : kWG;
     if (n_dev) {  // speculative: the count an earlier kernel wrote, n its bound
         const int nd = __builtin_amdgcn_readfirstlane(*n_dev);
@@ -1423,10 +1423,23 @@ __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void
     }
     Team<TEAM> tm;
     for (int i = tm.t; i < n; i += TEAM) {
+        // the set's pending kick (the parent call's split-K partials), as
+        // kick_reduce_kernel would have applied it
+        double kv[3] = {0.0, 0.0, 0.0};
+        if (kp.part) {
+            for (int k = 0; k < kp.K; ++k) {
+                const double *q = kp.part + 3 * (static_cast<size_t>(k) * kp.rows + i);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) kv[c] += q[c];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kv[c] = __dmul_rn(kv[c], kp.h);
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             a.wpos[3 * i + k] = total.pos[3 * i + k];
-            a.wvel[3 * i + k] = total.vel[3 * i + k];
+            a.wvel[3 * i + k] = kp.part ? __dadd_rn(total.vel[3 * i + k], kv[k])
+                                        : total.vel[3 * i + k];
         }
         a.wm[i] = total.m[i];
         a.wts[i] = total.timestep[i];
@@ -1507,9 +1520,12 @@ size_t subtree_workspace_bytes(int n_max) {
 
 int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
                    double eta, double eps2, int ts_mode, int max_level, void *ws,
-                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st, const int *n_dev) {
+                   size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st, const int *n_dev,
+                   const KickPending &kick) {
     HHS_REQUIRE(n > 0 && clevel >= 0 && max_level >= 0 && max_level <= kMaxFrames - 3 && acc_dev,
                 "subtree: n=%d, clevel=%d, max_level=%d", n, clevel, max_level);
+    HHS_REQUIRE(!kick.part || (!n_dev && kick.rows == n),
+                "subtree: a pending kick needs the host-known set size");
     // a speculative launch must pick the variant every count up to n picks
     HHS_REQUIRE(!n_dev || n <= kSubLds, "subtree: a device-side count needs n <= %d", kSubLds);
     const EnsLayout L = layout(1, n);
@@ -1549,7 +1565,7 @@ int launch_subtree(const Particles<double> &total, int n, int clevel, double dt,
                                 : (lds ? subtree_kernel<false, true, false>
                                        : subtree_kernel<false, false, false>));
     hipLaunchKernelGGL(kern, dim3(1), dim3(lds ? kSubTeam : kWG), 0, st, a, total, n, clevel, dt,
-                       calc, acc_dev, n_dev);
+                       calc, acc_dev, n_dev, kick);
     HHS_LAUNCH_CHECK("subtree_kernel");
     return HHS_OK;
 }

@@ -105,7 +105,27 @@ struct KickConfig {
 };
 
 KickConfig kick_pick_config(int n_sink, int n_src);
-template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t s);
+// A kick's chunk partials whose sum the next reader applies instead of a
+// reduce launch: rows [k][i][3] FP64 partial accelerations of K chunks,
+// added in chunk order from 0.0 and applied as vel = vel + T(a) * h with
+// no contraction (kick_reduce_kernel's arithmetic), so the bits are those
+// of the reduce.  part null: nothing pending.
+struct KickPending {
+    const double *part = nullptr;
+    int K = 0;
+    int rows = 0;
+    double h = 0.0;
+};
+// defer non-null: a split-K launch leaves its reduce to the caller, which
+// gets each sink group's pending partials (a: the first group, b: the
+// second) and must have them applied before anything reads those
+// velocities, and before the next kick launch on this host thread reuses
+// the partial buffer.
+struct KickDefer {
+    KickPending a, b;
+};
+template <typename T>
+int launch_kick(const KickArgs<T> &a, hipStream_t s, KickDefer *defer = nullptr);
 
 // -------------------------------------------------------------- timesteps
 template <typename T>
@@ -178,9 +198,12 @@ int launch_split_hold(const Particles<T> &total, const Particles<T> &slow,
 template <typename T>
 int launch_lone_fast_fixup(const Particles<T> &total, const Particles<T> &slow, int n, T h_slow,
                            hipStream_t st);
+// slow_kick: the slow set's pending kick (KickDefer), applied to its
+// velocities before the half-drift.
 template <typename T>
 int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int nf,
-                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s);
+                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s,
+                       const KickPending &slow_kick = KickPending{});
 template <typename T>
 int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
 
@@ -202,9 +225,11 @@ size_t subtree_workspace_bytes(int n_max);
 // (<= kSubtreeLevels - 3).  n_dev non-null: a speculative subtree whose
 // count is read on the device (n is then its upper bound, at most 256),
 // which runs only when that count is in [1, n].
+// kick: total's pending kick (KickDefer), applied to the velocities as the
+// subtree loads them.
 int launch_subtree(const Particles<double> &total, int n, int clevel, double dt, int calc,
                    double eta, double eps2, int ts_mode, int max_level, void *ws,
                    size_t ws_bytes, SubtreeAcc *acc_dev, hipStream_t st,
-                   const int *n_dev = nullptr);
+                   const int *n_dev = nullptr, const KickPending &kick = KickPending{});
 
 }  // namespace hhs

@@ -75,6 +75,7 @@ struct hhs_globals {
     int *next_n_dev = nullptr;
     int last_n[64];
     bool speculation = true;
+    bool fuse_reduce = true;  // kick partials applied by their readers
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
@@ -181,18 +182,19 @@ double guard_band(const hhs_globals *g) {
 // failure) are fetched by fetch_subtree_stats at the end of the hold_step
 // call.
 int queue_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
-                  bool calc, hipStream_t st, const int *n_dev = nullptr) {
+                  bool calc, hipStream_t st, const int *n_dev = nullptr,
+                  const KickPending &kick = KickPending{}) {
     if (!g->sub_pending) {
         HHS_HIP(hipMemsetAsync(g->sub_acc, 0, sizeof(SubtreeAcc), st));
         g->sub_pending = true;
     }
     return launch_subtree(total, n, clevel, dt, calc ? 1 : 0, g->eta, g->eps2, g->timestep_mode,
                           g->max_level - clevel, g->sub_ws, g->sub_ws_bytes, g->sub_acc, st,
-                          n_dev);
+                          n_dev, kick);
 }
 int run_subtree(hhs_globals *g, const Particles<double> &total, int n, int clevel, double dt,
-                bool calc, hipStream_t st) {
-    if (int rc = queue_subtree(g, total, n, clevel, dt, calc, st)) return rc;
+                bool calc, hipStream_t st, const KickPending &kick) {
+    if (int rc = queue_subtree(g, total, n, clevel, dt, calc, st, nullptr, kick)) return rc;
     g->simtime += dt;  // the leaves of a completed subtree tile its dt
     return HHS_OK;
 }
@@ -359,9 +361,11 @@ int speculate(hhs_globals *g, const Particles<T> &set, int n_bound, int clevel, 
     return HHS_OK;
 }
 
+// kick (FP64 sets): total's pending kick from the parent call's split-K
+// launch, when this call runs as a device subtree that applies it.
 template <typename T>
 int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt, bool calc,
-             hipStream_t st, Queued queued = Queued{}) {
+             hipStream_t st, Queued queued = Queued{}, const KickPending &kick = KickPending{}) {
     constexpr bool f64 = std::is_same<T, double>::value;
     if (clevel > g->max_level) {
         set_error("hold_step: recursion level %d exceeds max_level %d (timesteps collapsed?)",
@@ -372,7 +376,7 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
     hhs_hold_stats &S = g->stats;
     if constexpr (f64) {
         if (use_subtree(g, n, clevel)) {
-            if (!queued.subtree) return run_subtree(g, total, n, clevel, dt, calc, st);
+            if (!queued.subtree) return run_subtree(g, total, n, clevel, dt, calc, st, kick);
             g->simtime += dt;  // queued by the parent, and it ran with this n
             return HHS_OK;
         }
@@ -416,6 +420,7 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
     const int nf = raw == 1 ? 0 : raw;
     const int ns = n - nf;
     if (raw == 1 && (rc = launch_lone_fast_fixup<T>(total, slow, n, h, st))) return rc;
+    KickDefer kdefer;  // the kick's partials left to their readers (below)
 
     // src/hold.cuh:106-114
     if (nf == 0) g->simtime += static_cast<double>(dt);
@@ -440,7 +445,15 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
             ks.sb_acc = fast.acc;
             ks.n_sink_b = nf;
         }
-        if ((rc = launch_kick<T>(ks, st))) return rc;
+        // When the second fast recursion is a device subtree (or there is
+        // none), no host kick runs before this call's scatter-back, so the
+        // kick's split-K partials may wait for their readers: the fast
+        // sinks' sum is applied by that subtree as it loads the set, the
+        // slow sinks' by the scatter-back (one reduce launch fewer).
+        bool defer = false;
+        if constexpr (f64)
+            defer = !exact && g->fuse_reduce && (nf == 0 || use_subtree(g, nf, clevel + 1));
+        if ((rc = launch_kick<T>(ks, st, defer ? &kdefer : nullptr))) return rc;
         if (exact && nf > 0) {
             KickArgs<T> ksf{slow.pos, slow.vel, slow.acc, ns, fast.pos, fast.m, nf, nullptr,
                             nullptr,  0,        dt,       g->eps2,  prec};
@@ -453,10 +466,12 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
         // src/hold.cuh:158-159: done by the scatter-back below
     }
     // src/hold.cuh:168-173
-    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), true, st))) return rc;
+    if (nf > 0 &&
+        (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), true, st, Queued{}, kdefer.b)))
+        return rc;
     // src/hold.cuh:177-178 (the deeper levels reuse the level's sets, so
     // scatter back now), with the second slow half-drift
-    return launch_update_pair<T>(total, fast, nf, slow, ns, h, st);
+    return launch_update_pair<T>(total, fast, nf, slow, ns, h, st, kdefer.a);
 }
 
 // hold_step on one particle set of either precision (the C entry points).
@@ -604,6 +619,12 @@ int hhs_globals_subtree_max(const hhs_globals *g) { return g ? g->subtree_max : 
 int hhs_globals_set_speculation(hhs_globals *g, int on) {
     HHS_REQUIRE(g, "set_speculation: null globals");
     g->speculation = on != 0;
+    return HHS_OK;
+}
+
+int hhs_globals_set_fused_reduce(hhs_globals *g, int on) {
+    HHS_REQUIRE(g, "set_fused_reduce: null globals");
+    g->fuse_reduce = on != 0;
     return HHS_OK;
 }
 

@@ -380,13 +380,6 @@ int launch_swp(const KickArgs<T> &a, SplitK K, double *partial, const float *mf,
                            a.s0_pos, a.s0_m, a.n0, a.s1_pos, a.s1_m, a.n1, a.h, eps2f, eps2, K.a,
                            K.b, part, mf);
     HHS_LAUNCH_CHECK("kick_kernel");
-    if (split) {
-        const int rows = (K.a > 1 ? a.n_sink : 0) + (K.b > 1 ? a.n_sink_b : 0);
-        hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((rows + 255) / 256), dim3(256), 0, st,
-                           a.si_vel, a.si_acc, a.n_sink, a.sb_vel, a.sb_acc, a.n_sink_b, part,
-                           K.a, K.b, a.h);
-        HHS_LAUNCH_CHECK("kick_reduce_kernel");
-    }
     return HHS_OK;
 }
 
@@ -474,7 +467,8 @@ int pick_split(int n_sink, int n_src, int S, int W) {
     return K;
 }
 
-template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
+template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st, KickDefer *defer) {
+    if (defer) *defer = KickDefer{};
     HHS_REQUIRE(a.n_sink >= 0 && a.n_sink_b >= 0 && a.n0 >= 0 && a.n1 >= 0,
                 "kick: negative size");
     const int n_rows = a.n_sink + a.n_sink_b;  // sinks of both groups
@@ -540,13 +534,29 @@ template <typename T> int launch_kick(const KickArgs<T> &a, hipStream_t st) {
     case 4: rc = launch_s<T, 4>(a, c.waves, P, K, partial, mf, st); break;
     default: set_error("kick: unsupported sinks per lane %d", c.sinks_per_lane);
     }
-    if (rc == HHS_OK && (K.a > 1 || K.b > 1)) rc = scratch.release(st);
+    if (rc == HHS_OK && (K.a > 1 || K.b > 1)) {
+        if (defer) {
+            // the readers add the partials (the layout kick_reduce_kernel reads)
+            if (K.a > 1) defer->a = KickPending{partial, K.a, a.n_sink, static_cast<double>(a.h)};
+            if (K.b > 1)
+                defer->b = KickPending{partial + (K.a > 1 ? 3 * static_cast<size_t>(K.a) * a.n_sink
+                                                          : 0),
+                                       K.b, a.n_sink_b, static_cast<double>(a.h)};
+        } else {
+            const int rows = (K.a > 1 ? a.n_sink : 0) + (K.b > 1 ? a.n_sink_b : 0);
+            hipLaunchKernelGGL((kick_reduce_kernel<T>), dim3((rows + 255) / 256), dim3(256), 0,
+                               st, a.si_vel, a.si_acc, a.n_sink, a.sb_vel, a.sb_acc, a.n_sink_b,
+                               partial, K.a, K.b, a.h);
+            HHS_LAUNCH_CHECK("kick_reduce_kernel");
+        }
+        rc = scratch.release(st);
+    }
     if (rc == HHS_OK && conv) rc = mass_scratch.release(st);
     return rc;
 }
 
-template int launch_kick<double>(const KickArgs<double> &, hipStream_t);
-template int launch_kick<float>(const KickArgs<float> &, hipStream_t);
+template int launch_kick<double>(const KickArgs<double> &, hipStream_t, KickDefer *);
+template int launch_kick<float>(const KickArgs<float> &, hipStream_t, KickDefer *);
 
 }  // namespace hhs
 

@@ -50,10 +50,12 @@ __global__ __launch_bounds__(256) void update_kernel(Particles<T> prev, Particle
 // expression).  The fast recursion that runs between the drift and the
 // scatter-back never reads the slow set, so deferring the drift to here
 // changes no value.
+// kp: the slow set's pending kick, applied first as kick_reduce_kernel
+// applies it (chunk partials added in order from 0.0, vel + T(a) * h).
 template <typename T>
 __global__ __launch_bounds__(256) void update_pair_kernel(Particles<T> prev, Particles<T> fast,
                                                           int nf, Particles<T> slow, int ns,
-                                                          T h) {
+                                                          T h, KickPending kp) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf + ns) return;
@@ -62,8 +64,21 @@ __global__ __launch_bounds__(256) void update_pair_kernel(Particles<T> prev, Par
     const int i = is_fast ? t : t - nf;
     const int p = c.parent_id[i];
     T x = c.pos[3 * i + 0], y = c.pos[3 * i + 1], z = c.pos[3 * i + 2];
-    const T vx = c.vel[3 * i + 0], vy = c.vel[3 * i + 1], vz = c.vel[3 * i + 2];
+    T vx = c.vel[3 * i + 0], vy = c.vel[3 * i + 1], vz = c.vel[3 * i + 2];
     const T m = c.m[i], ts = c.timestep[i];
+    if (!is_fast && kp.part) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int k = 0; k < kp.K; ++k) {
+            const double *q = kp.part + 3 * (static_cast<size_t>(k) * kp.rows + i);
+            ax += q[0];
+            ay += q[1];
+            az += q[2];
+        }
+        const T hk = static_cast<T>(kp.h);
+        vx = vx + static_cast<T>(ax) * hk;
+        vy = vy + static_cast<T>(ay) * hk;
+        vz = vz + static_cast<T>(az) * hk;
+    }
     if (!is_fast) {
         x = x + vx * h;
         y = y + vy * h;
@@ -176,20 +191,24 @@ template int launch_update<float>(const Particles<float> &, const Particles<floa
 
 template <typename T>
 int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int nf,
-                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s) {
+                       const Particles<T> &slow, int ns, T h_slow, hipStream_t s,
+                       const KickPending &slow_kick) {
     HHS_REQUIRE(nf >= 0 && ns >= 0, "update_pair: negative size");
     const int n = nf + ns;
     if (n == 0) return HHS_OK;
+    HHS_REQUIRE(!slow_kick.part || slow_kick.rows == ns, "update_pair: pending kick of %d rows",
+                slow_kick.rows);
     hipLaunchKernelGGL((update_pair_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s, prev,
-                       fast, nf, slow, ns, h_slow);
+                       fast, nf, slow, ns, h_slow, slow_kick);
     HHS_LAUNCH_CHECK("update_pair_kernel");
     return HHS_OK;
 }
 template int launch_update_pair<double>(const Particles<double> &, const Particles<double> &,
                                         int, const Particles<double> &, int, double,
-                                        hipStream_t);
+                                        hipStream_t, const KickPending &);
 template int launch_update_pair<float>(const Particles<float> &, const Particles<float> &, int,
-                                       const Particles<float> &, int, float, hipStream_t);
+                                       const Particles<float> &, int, float, hipStream_t,
+                                       const KickPending &);
 
 }  // namespace hhs
 

@@ -312,3 +312,33 @@ def test_hold_speculation_same_bits(hhs, torch_cuda, case):
                     g.simtime))
     assert out[0][2]["calls"] > 3
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("case", ["default", "no_speculation", "eps0", "root_step_16k"])
+def test_hold_fused_reduce_same_bits(hhs, torch_cuda, case):
+    """A call's split-K kick partials added by their readers -- the device
+    subtree of its second fast recursion (fast sinks) and its scatter-back
+    (slow sinks) -- instead of a reduce launch: positions, velocities,
+    timesteps, statistics and simtime are the same bits as with the reduce
+    launch, over several root steps; also with eps = 0 (unsoftened kicks),
+    without speculative launches, and on one root step of the N = 16384
+    sphere."""
+    n, eps, t_end, dt = {"default": (3000, 1e-3, 0.125, 1.0 / 16),
+                         "no_speculation": (2000, 1e-3, 0.125, 1.0 / 16),
+                         "eps0": (1500, 0.0, 0.125, 1.0 / 16),
+                         "root_step_16k": (16384, 1e-4, 0.125, 0.125)}[case]
+    kw = dict(fast_timesteps=hhs.TIMESTEP_FP32)
+    if case == "no_speculation":
+        kw["speculation"] = False
+    pos, vel, m = plummer(hhs, n, seed=7500 + n, eps2=eps * eps)
+    out = []
+    for fused in (True, False):
+        d = hhs.DeviceParticles.from_host(pos, vel, m)
+        g = hhs.Globals(n, 0.01, eps, max_level=40, fused_reduce=fused, **kw)
+        g.hold(d, t_end, dt)
+        torch_cuda.cuda.synchronize()
+        out.append((d.pos.cpu().numpy().tobytes(), d.vel.cpu().numpy().tobytes(),
+                    d.timestep.cpu().numpy().tobytes(), g.stats(), g.simtime))
+        g.close()
+    assert out[0][3]["calls"] > 3
+    assert out[0] == out[1]
