@@ -368,6 +368,11 @@ int hhs_globals_set_speculation(hhs_globals *g, int on);
  * scatter-back (slow sinks) instead of a reduce launch.  Results are the
  * same bits either way.  No reference counterpart. */
 int hhs_globals_set_fused_reduce(hhs_globals *g, int on);
+/* Chained scatter-backs (default on): a call that is its parent's second
+ * fast recursion leaves its scatter-back to the parent, whose own follows
+ * at once, and the chain of them runs as one launch (up to 8 levels).
+ * Results are the same bits either way.  No reference counterpart. */
+int hhs_globals_set_chain_scatter(hhs_globals *g, int on);
 int hhs_globals_timestep_mode(const hhs_globals *g);
 int hhs_globals_kick_precision(const hhs_globals *g);
 /* Root steps completed by hhs_evolve_f64 (restored from a snapshot on

@@ -206,6 +206,30 @@ int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int n
                        const KickPending &slow_kick = KickPending{});
 template <typename T>
 int launch_update(const Particles<T> &prev, const Particles<T> &cur, int n, hipStream_t s);
+// Scatter-backs of a chain of HOLD calls, each the second fast recursion of
+// the one above it, in one launch.  Level 0 is the top call, whose own set
+// receives every element; level k's set is level k-1's fast set.  Each
+// level's slow records are drifted by its h after its pending kick, as its
+// own scatter-back would; the deepest level's fast records are copied.  An
+// element of level k's slow set goes to lv[k].slow.parent_id, then up
+// through lv[k-1 .. 0].fast_pid.  The same bits as the calls' scatter-backs
+// one after the other, deepest first.
+constexpr int kScatterChainMax = 8;
+template <typename T> struct ScatterLevel {
+    Particles<T> slow;
+    int ns;
+    T h;
+    KickPending kick;
+    const int *fast_pid;  // parent_id of this level's fast set
+    int nf;
+};
+template <typename T> struct ScatterChain {
+    ScatterLevel<T> lv[kScatterChainMax];
+    Particles<T> bottom_fast;  // the deepest level's fast set
+    int depth = 0;
+};
+template <typename T>
+int launch_update_chain(const Particles<T> &prev, const ScatterChain<T> &c, hipStream_t s);
 
 // ------------------------------------------------- device-side subtrees
 // Statistics of the device subtrees run since the accumulator was cleared,

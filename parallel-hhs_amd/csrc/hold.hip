@@ -76,6 +76,7 @@ struct hhs_globals {
     int last_n[64];
     bool speculation = true;
     bool fuse_reduce = true;  // kick partials applied by their readers
+    bool chain_scatter = true;  // chained scatter-backs in one launch
     std::vector<hhs::LevelBuffers> levels;
     double simtime = 0.0;
     hhs_hold_stats stats{};
@@ -363,9 +364,14 @@ int speculate(hhs_globals *g, const Particles<T> &set, int n_bound, int clevel, 
 
 // kick (FP64 sets): total's pending kick from the parent call's split-K
 // launch, when this call runs as a device subtree that applies it.
+// up: this call is its parent's second fast recursion, whose scatter-back
+// comes right after it; a host call then hands its own scatter-back (and
+// those deferred to it) up in *up instead of launching it, and the parent
+// launches the whole chain at once (one launch for the lot).
 template <typename T>
 int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt, bool calc,
-             hipStream_t st, Queued queued = Queued{}, const KickPending &kick = KickPending{}) {
+             hipStream_t st, Queued queued = Queued{}, const KickPending &kick = KickPending{},
+             ScatterChain<T> *up = nullptr) {
     constexpr bool f64 = std::is_same<T, double>::value;
     if (clevel > g->max_level) {
         set_error("hold_step: recursion level %d exceeds max_level %d (timesteps collapsed?)",
@@ -466,12 +472,30 @@ int hold_rec(hhs_globals *g, const Particles<T> &total, int n, int clevel, T dt,
         // src/hold.cuh:158-159: done by the scatter-back below
     }
     // src/hold.cuh:168-173
-    if (nf > 0 &&
-        (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), true, st, Queued{}, kdefer.b)))
+    ScatterChain<T> below;  // scatter-backs the second recursion left to this call
+    if (nf > 0 && (rc = hold_rec<T>(g, fast, nf, clevel + 1, dt / T(2), true, st, Queued{},
+                                    kdefer.b, g->chain_scatter ? &below : nullptr)))
         return rc;
     // src/hold.cuh:177-178 (the deeper levels reuse the level's sets, so
     // scatter back now), with the second slow half-drift
-    return launch_update_pair<T>(total, fast, nf, slow, ns, h, st, kdefer.a);
+    if (below.depth == 0 && !up)
+        return launch_update_pair<T>(total, fast, nf, slow, ns, h, st, kdefer.a);
+    ScatterChain<T> ch;
+    ch.lv[0] = ScatterLevel<T>{slow, ns, h, kdefer.a, fast.parent_id, nf};
+    ch.bottom_fast = fast;
+    ch.depth = 1;
+    if (below.depth > 0) {
+        for (int k = 0; k < below.depth; ++k) ch.lv[1 + k] = below.lv[k];
+        ch.bottom_fast = below.bottom_fast;
+        ch.depth = 1 + below.depth;
+    }
+    // the parent prepends its own level, so there must be room for it
+    if (up && ch.depth < kScatterChainMax) {
+        *up = ch;
+        return HHS_OK;
+    }
+    if (ch.depth == 1) return launch_update_pair<T>(total, fast, nf, slow, ns, h, st, kdefer.a);
+    return launch_update_chain<T>(total, ch, st);
 }
 
 // hold_step on one particle set of either precision (the C entry points).
@@ -619,6 +643,12 @@ int hhs_globals_subtree_max(const hhs_globals *g) { return g ? g->subtree_max : 
 int hhs_globals_set_speculation(hhs_globals *g, int on) {
     HHS_REQUIRE(g, "set_speculation: null globals");
     g->speculation = on != 0;
+    return HHS_OK;
+}
+
+int hhs_globals_set_chain_scatter(hhs_globals *g, int on) {
+    HHS_REQUIRE(g, "set_chain_scatter: null globals");
+    g->chain_scatter = on != 0;
     return HHS_OK;
 }
 

@@ -94,6 +94,60 @@ __global__ __launch_bounds__(256) void update_pair_kernel(Particles<T> prev, Par
     prev.timestep[p] = ts;
 }
 
+// The scatter-backs of a chain of calls in one launch (ScatterChain): thread
+// t takes element t of the concatenation slow_0, slow_1, ..., slow_{d-1},
+// fast_{d-1} (their sizes add up to the top call's set), applies its level's
+// pending kick and half-drift as update_pair_kernel does, and writes it to
+// the top call's set through the parent_id maps of the levels above it.
+template <typename T>
+__global__ __launch_bounds__(256) void update_chain_kernel(Particles<T> prev, ScatterChain<T> c,
+                                                           int n) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int k = 0, base = 0;
+    while (k < c.depth && t >= base + c.lv[k].ns) {
+        base += c.lv[k].ns;
+        ++k;
+    }
+    const bool slow = k < c.depth;
+    const int lvl = slow ? k : c.depth - 1;
+    const Particles<T> &src = slow ? c.lv[k].slow : c.bottom_fast;
+    const int i = t - base;
+    int p = src.parent_id[i];
+    for (int m = lvl - 1; m >= 0; --m) p = c.lv[m].fast_pid[p];
+    T x = src.pos[3 * i + 0], y = src.pos[3 * i + 1], z = src.pos[3 * i + 2];
+    T vx = src.vel[3 * i + 0], vy = src.vel[3 * i + 1], vz = src.vel[3 * i + 2];
+    if (slow) {
+        const KickPending kp = c.lv[k].kick;
+        if (kp.part) {
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int q = 0; q < kp.K; ++q) {
+                const double *r = kp.part + 3 * (static_cast<size_t>(q) * kp.rows + i);
+                ax += r[0];
+                ay += r[1];
+                az += r[2];
+            }
+            const T hk = static_cast<T>(kp.h);
+            vx = vx + static_cast<T>(ax) * hk;
+            vy = vy + static_cast<T>(ay) * hk;
+            vz = vz + static_cast<T>(az) * hk;
+        }
+        const T h = c.lv[k].h;
+        x = x + vx * h;
+        y = y + vy * h;
+        z = z + vz * h;
+    }
+    prev.pos[3 * p + 0] = x;
+    prev.pos[3 * p + 1] = y;
+    prev.pos[3 * p + 2] = z;
+    prev.vel[3 * p + 0] = vx;
+    prev.vel[3 * p + 1] = vy;
+    prev.vel[3 * p + 2] = vz;
+    prev.m[p] = src.m[i];
+    prev.timestep[p] = src.timestep[i];
+}
+
 // Row sums of the potential: row[i] = m_i * sum_{j>i} m_j / sqrt(r^2+eps2),
 // j ascending (the order of a serial CPU loop), plus the kinetic term of i.
 // The j loop is wave-uniform (scalar loads); lanes with j <= i skip.
@@ -203,6 +257,32 @@ int launch_update_pair(const Particles<T> &prev, const Particles<T> &fast, int n
     HHS_LAUNCH_CHECK("update_pair_kernel");
     return HHS_OK;
 }
+template <typename T>
+int launch_update_chain(const Particles<T> &prev, const ScatterChain<T> &c, hipStream_t s) {
+    HHS_REQUIRE(c.depth >= 1 && c.depth <= kScatterChainMax, "update_chain: depth %d", c.depth);
+    long long n = 0;
+    for (int k = 0; k < c.depth; ++k) {
+        HHS_REQUIRE(c.lv[k].ns >= 0 && c.lv[k].nf >= 0, "update_chain: negative size");
+        HHS_REQUIRE(!c.lv[k].kick.part || c.lv[k].kick.rows == c.lv[k].ns,
+                    "update_chain: pending kick of %d rows", c.lv[k].kick.rows);
+        // each level's set is the slow set plus the next level's set
+        if (k + 1 < c.depth)
+            HHS_REQUIRE(c.lv[k].nf == c.lv[k + 1].ns + c.lv[k + 1].nf,
+                        "update_chain: level %d has %d fast records, level %d holds %d", k,
+                        c.lv[k].nf, k + 1, c.lv[k + 1].ns + c.lv[k + 1].nf);
+        n += c.lv[k].ns;
+    }
+    n += c.lv[c.depth - 1].nf;
+    if (n == 0) return HHS_OK;
+    hipLaunchKernelGGL((update_chain_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)),
+                       dim3(256), 0, s, prev, c, static_cast<int>(n));
+    HHS_LAUNCH_CHECK("update_chain_kernel");
+    return HHS_OK;
+}
+template int launch_update_chain<double>(const Particles<double> &,
+                                         const ScatterChain<double> &, hipStream_t);
+template int launch_update_chain<float>(const Particles<float> &, const ScatterChain<float> &,
+                                        hipStream_t);
 template int launch_update_pair<double>(const Particles<double> &, const Particles<double> &,
                                         int, const Particles<double> &, int, double,
                                         hipStream_t, const KickPending &);

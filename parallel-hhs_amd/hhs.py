@@ -180,6 +180,7 @@ _SIGS = {
     "hhs_globals_subtree_max": (_i, [_vp]),
     "hhs_globals_set_speculation": (_i, [_vp, _i]),
     "hhs_globals_set_fused_reduce": (_i, [_vp, _i]),
+    "hhs_globals_set_chain_scatter": (_i, [_vp, _i]),
     "hhs_globals_timestep_mode": (_i, [_vp]),
     "hhs_globals_kick_precision": (_i, [_vp]),
     "hhs_globals_root_steps": (C.c_longlong, [_vp]),
@@ -489,14 +490,16 @@ class Globals:
     def __init__(self, n_total: int, eta: float, eps: float, max_level: int = 48,
                  fast_timesteps: bool = False, kick_precision: int = -1,
                  subtree_max: int | None = None, speculation: bool = True,
-                 fused_reduce: bool = True):
+                 fused_reduce: bool = True, chain_scatter: bool = True):
         """`kick_precision` is a KICK_* mode for this HOLD (-1: the process-wide
         setting).  kick_precision=KICK_F64_EXACT with exact timesteps is the
         FP64-only HOLD mode, bit-exact with the FP64 CPU HOLD.  `speculation`
         queues each call's first fast recursion before its counts are read
         back (same results, fewer idle gaps).  `fused_reduce` lets a call's
         device subtree and scatter-back add its kick's split-K partials
-        (same results, one launch fewer per such call)."""
+        (same results, one launch fewer per such call).  `chain_scatter`
+        runs the scatter-backs of a chain of second fast recursions as one
+        launch (same results)."""
         h = C.c_void_p()
         check(_lib.hhs_globals_create(C.byref(h), n_total, eta, eps, max_level))
         self.h = h
@@ -510,6 +513,8 @@ class Globals:
             check(_lib.hhs_globals_set_speculation(h, 0))
         if not fused_reduce:
             check(_lib.hhs_globals_set_fused_reduce(h, 0))
+        if not chain_scatter:
+            check(_lib.hhs_globals_set_chain_scatter(h, 0))
 
     @property
     def subtree_max(self) -> int:

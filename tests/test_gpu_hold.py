@@ -342,3 +342,40 @@ def test_hold_fused_reduce_same_bits(hhs, torch_cuda, case):
         g.close()
     assert out[0][3]["calls"] > 3
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("case", ["default", "no_subtree", "fp32_sets", "no_fused_reduce",
+                                  "fp64_exact", "root_step_16k"])
+def test_hold_chain_scatter_same_bits(hhs, torch_cuda, case):
+    """The scatter-backs of a chain of calls, each its parent's second fast
+    recursion, run as one launch: positions, velocities, timesteps,
+    statistics and simtime are the same bits as with one scatter-back launch
+    per call, over several root steps; with and without device subtrees
+    (the host recursion then reaches the deepest levels, and chains longer
+    than one launch holds), on FP32 sets, without the fused kick reduce, in
+    the FP64-only mode, and on one root step of the N = 16384 sphere."""
+    import torch
+    n, eps, t_end, dt = 2500, 1e-3, 0.125, 1.0 / 16
+    kw, dtype = dict(fast_timesteps=hhs.TIMESTEP_FP32), torch.float64
+    if case == "no_subtree":
+        kw["subtree_max"] = 0
+    elif case == "fp32_sets":
+        kw, dtype = {}, torch.float32
+    elif case == "no_fused_reduce":
+        kw["fused_reduce"] = False
+    elif case == "fp64_exact":
+        n, kw = 600, dict(kick_precision=hhs.KICK_F64_EXACT)
+    elif case == "root_step_16k":
+        n, eps, t_end, dt = 16384, 1e-4, 0.125, 0.125
+    pos, vel, m = plummer(hhs, n, seed=7600 + n, eps2=eps * eps)
+    out = []
+    for chain in (True, False):
+        d = hhs.DeviceParticles.from_host(pos, vel, m, dtype=dtype)
+        g = hhs.Globals(n, 0.01, eps, max_level=40, chain_scatter=chain, **kw)
+        g.hold(d, t_end, dt)
+        torch_cuda.cuda.synchronize()
+        out.append((d.pos.cpu().numpy().tobytes(), d.vel.cpu().numpy().tobytes(),
+                    d.timestep.cpu().numpy().tobytes(), g.stats(), g.simtime))
+        g.close()
+    assert out[0][3]["calls"] > 3
+    assert out[0] == out[1]
