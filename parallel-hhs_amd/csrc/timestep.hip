@@ -383,6 +383,7 @@ __global__ __launch_bounds__(256) void pack_ds_kernel(const double *__restrict__
     if (i == 0 && gfix_hdr) {
         gfix_hdr[0] = 0;  // flagged timesteps
         gfix_hdr[1] = 0;  // more than kGuardFixCap of them
+        gfix_hdr[2] = 0;  // guard_partial_kernel workgroups done
     }
     if (i >= n) return;
     float v[kDsRec];
@@ -513,19 +514,19 @@ __global__ __launch_bounds__(256) void timesteps_sym32_kernel(const float *__res
 // symmetric kernel the flagged timesteps are instead listed by the reduce
 // kernel and recomputed by the whole GPU: guard_partial_kernel gives each
 // workgroup 256 sources and every flagged particle one exact pair per
-// thread (workgroup minimum per particle), guard_final_kernel takes the
-// minimum over the workgroups and writes it into ts.  A minimum does not
-// depend on order, so the values are the guard's bit for bit.  More than
-// kGuardFixCap flagged timesteps set the overflow word, and the split then
-// runs its own guard as before (on every flagged timestep; the fixed ones
-// give the same values again).
+// thread (workgroup minimum per particle), and its last workgroup to finish
+// takes the minimum over the workgroups and writes it into ts.  A minimum
+// does not depend on order, so the values are the guard's bit for bit.  More
+// than kGuardFixCap flagged timesteps set the overflow word (in the reduce
+// kernel), and the split then runs its own guard as before (on every flagged
+// timestep; the fixed ones give the same values again).
 constexpr int kGuardFixCap = 256;
 
 struct GuardFix {
     int on;
     TsGuard g;
     double eta, c_ff, eps2;
-    int *hdr;          // [0] flagged count, [1] overflow
+    int *hdr;          // [0] flagged count, [1] overflow, [2] guard workgroups done
     int *list;         // flagged particles (n entries)
     double *partial;   // [kGuardFixCap][groups]
 };
@@ -563,17 +564,25 @@ __global__ __launch_bounds__(64 * kSymRedSlices) void timesteps_sym32_reduce_ker
     if (gf.on && guard_near_pivot(gf.g, v)) {
         const int k = atomicAdd(gf.hdr, 1);
         if (k < n) gf.list[k] = i;
+        if (k >= kGuardFixCap) gf.hdr[1] = 1;  // the split runs its own guard
     }
 }
 
-// Workgroup g: sources [256 g, 256 g + 256) against every listed particle.
+// Workgroup g: sources [256 g, 256 g + 256) against every listed particle;
+// the last workgroup to finish then takes each listed particle's minimum
+// over the workgroups and writes it into ts.  No timestep is listed in most
+// calls: every workgroup then returns at once, with no atomic and no fence.
+// Otherwise each workgroup's partials are released (device scope) before
+// it counts itself done, and the last one acquires before it reads them.
 __global__ __launch_bounds__(256) void guard_partial_kernel(const double *__restrict__ pos,
                                                             const double *__restrict__ vel,
                                                             const double *__restrict__ m, int n,
-                                                            GuardFix gf) {
+                                                            double *__restrict__ ts, GuardFix gf) {
     __shared__ double s_min[4];
+    __shared__ int s_last;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int cnt = min(__builtin_amdgcn_readfirstlane(gf.hdr[0]), kGuardFixCap);
+    if (cnt == 0) return;
     const int j = blockIdx.x * 256 + t;
     for (int f = 0; f < cnt; ++f) {
         const int s = __builtin_amdgcn_readfirstlane(gf.list[f]);
@@ -591,15 +600,14 @@ __global__ __launch_bounds__(256) void guard_partial_kernel(const double *__rest
                 vmin(vmin(s_min[0], s_min[1]), vmin(s_min[2], s_min[3]));
         __syncthreads();
     }
-}
-
-// One workgroup: the listed particles' exact timesteps into ts.
-__global__ __launch_bounds__(256) void guard_final_kernel(double *__restrict__ ts, int groups,
-                                                          GuardFix gf) {
-    __shared__ double s_min[4];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int total = __builtin_amdgcn_readfirstlane(gf.hdr[0]);
-    const int cnt = min(total, kGuardFixCap);
+    if (t == 0) {
+        __threadfence();  // this workgroup's partials, before it counts itself done
+        s_last = atomicAdd(gf.hdr + 2, 1) == static_cast<int>(gridDim.x) - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // every workgroup's partials
+    const int groups = static_cast<int>(gridDim.x);
     for (int f = 0; f < cnt; ++f) {
         double best = TsConst<double>::huge;
         for (int c = t; c < groups; c += 256)
@@ -610,7 +618,6 @@ __global__ __launch_bounds__(256) void guard_final_kernel(double *__restrict__ t
         if (t == 0) ts[gf.list[f]] = vmin(vmin(s_min[0], s_min[1]), vmin(s_min[2], s_min[3]));
         __syncthreads();
     }
-    if (t == 0) gf.hdr[1] = total > kGuardFixCap ? 1 : 0;
 }
 
 template <typename V>
@@ -768,11 +775,8 @@ int launch_timesteps_sym(int mode, const double *pos, const double *vel, const d
         }
         HHS_LAUNCH_CHECK("timesteps_sym32_kernel");
         if (gf.on) {
-            hipLaunchKernelGGL(guard_partial_kernel, red, dim3(256), 0, s, pos, vel, m, n, gf);
+            hipLaunchKernelGGL(guard_partial_kernel, red, dim3(256), 0, s, pos, vel, m, n, ts, gf);
             HHS_LAUNCH_CHECK("guard_partial_kernel");
-            hipLaunchKernelGGL(guard_final_kernel, dim3(1), dim3(256), 0, s, ts,
-                               static_cast<int>(red.x), gf);
-            HHS_LAUNCH_CHECK("guard_final_kernel");
             *guard_overflow = gf.hdr + 1;
         }
     } else {

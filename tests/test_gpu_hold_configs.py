@@ -89,6 +89,29 @@ def test_guard_host_recursion_equals_exact(hhs, torch_cuda, n, eps, mode):
     _assert_same_bits(runs)
 
 
+def test_guard_stage_after_symmetric_equals_exact(hhs, torch_cuda):
+    """From 8192 particles the FP32 timesteps come from the symmetric kernel,
+    and its guard stage (flagged list, per-workgroup exact partials, the last
+    workgroup's minima) replaces the near-pivot timesteps: with exact kicks
+    the run gives the bits of exact timesteps over several root steps."""
+    n, eps, eta = 9000, 1e-4, 0.01
+    pos, vel, m = plummer(hhs, n, seed=7150, eps2=eps * eps)
+    runs = _pair_runs(hhs, pos, vel, m, eta, eps, 1.0 / 64, 1.0 / 32,
+                      [hhs.TIMESTEP_FP32, hhs.TIMESTEP_EXACT], hhs.KICK_F64_EXACT)
+    torch_cuda.cuda.synchronize()
+    _assert_same_bits(runs)
+    # a root dt between one particle's FP32 and exact timesteps, so that the
+    # stage has at least that particle to fix in the root call
+    t64 = _ts(hhs, torch_cuda, pos, vel, m, eta, eps * eps, hhs.TIMESTEP_EXACT)
+    t32 = _ts(hhs, torch_cuda, pos, vel, m, eta, eps * eps, hhs.TIMESTEP_FP32, True)
+    dt, i = _straddle_dt(t32, t64)
+    assert (t32[i] <= dt) != (t64[i] <= dt)
+    runs = _pair_runs(hhs, pos, vel, m, eta, eps, dt, dt, [hhs.TIMESTEP_FP32, hhs.TIMESTEP_EXACT],
+                      hhs.KICK_F64_EXACT)
+    torch_cuda.cuda.synchronize()
+    _assert_same_bits(runs)
+
+
 def _straddle_dt(t32, t64):
     """A dt between the FP32 and exact timesteps of the particle where they
     differ most (relative), so that an unguarded FP32 split decides that
