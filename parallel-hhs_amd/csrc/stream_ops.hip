@@ -6,6 +6,8 @@
 // particle); they run without FMA contraction so FP64 results are bit-exact
 // against the same expression on the CPU.
 #include <cmath>
+#include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "hhs_internal.h"
@@ -14,12 +16,41 @@ namespace hhs {
 namespace {
 
 template <typename T>
-__global__ __launch_bounds__(256) void drift_kernel(T *__restrict__ pos,
-                                                    const T *__restrict__ vel, int n3, T h) {
+__global__ __launch_bounds__(256) void drift_scalar_kernel(T *__restrict__ pos,
+                                                           const T *__restrict__ vel, int n3,
+                                                           T h) {
 #pragma clang fp contract(off)
     const int stride = gridDim.x * blockDim.x;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += stride)
         pos[k] = pos[k] + vel[k] * h;  // src/drift.cuh:25
+}
+
+// Each thread moves 16 bytes of pos and of vel per step (one component
+// pair of FP64, or four of FP32, as one vector load), so that a wave keeps
+// 1 KB of each array in flight per access; the element expression is the
+// scalar one.  For 16-byte aligned arrays (drift_scalar_kernel otherwise);
+// an odd tail of components is taken by the first thread.
+template <typename T>
+__global__ __launch_bounds__(256) void drift_kernel(T *__restrict__ pos,
+                                                    const T *__restrict__ vel, int n3, T h) {
+#pragma clang fp contract(off)
+    constexpr int V = 16 / sizeof(T);  // components per 16-byte vector
+    using Vec = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    const int nv = n3 / V;
+    const int stride = gridDim.x * blockDim.x;
+    Vec *pv = reinterpret_cast<Vec *>(pos);
+    const Vec *vv = reinterpret_cast<const Vec *>(vel);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += stride) {
+        Vec p = pv[k];
+        const Vec v = vv[k];
+        T *pe = reinterpret_cast<T *>(&p);
+        const T *ve = reinterpret_cast<const T *>(&v);
+#pragma unroll
+        for (int c = 0; c < V; ++c) pe[c] = pe[c] + ve[c] * h;  // src/drift.cuh:25
+        pv[k] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int k = nv * V; k < n3; ++k) pos[k] = pos[k] + vel[k] * h;
 }
 
 template <typename T>
@@ -217,8 +248,19 @@ template <typename T> int launch_drift(T *pos, const T *vel, int n, T h, hipStre
     if (n == 0) return HHS_OK;
     HHS_REQUIRE(pos && vel, "drift: null array");
     const int n3 = 3 * n;
-    int blocks = (n3 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    if (reinterpret_cast<uintptr_t>(pos) % 16 != 0 || reinterpret_cast<uintptr_t>(vel) % 16 != 0) {
+        int blocks = (n3 + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL((drift_scalar_kernel<T>), dim3(blocks), dim3(256), 0, s, pos, vel, n3,
+                           h);
+        HHS_LAUNCH_CHECK("drift_scalar_kernel");
+        return HHS_OK;
+    }
+    // one vector per thread: a capped grid left part of the last grid-stride
+    // pass on half the threads (2^24 particles: 0.66 of HBM against 0.89 at
+    // 2^22)
+    const int nv = n3 / static_cast<int>(16 / sizeof(T));
+    const int blocks = nv > 0 ? (nv + 255) / 256 : 1;
     hipLaunchKernelGGL((drift_kernel<T>), dim3(blocks), dim3(256), 0, s, pos, vel, n3, h);
     HHS_LAUNCH_CHECK("drift_kernel");
     return HHS_OK;

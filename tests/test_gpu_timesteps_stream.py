@@ -110,9 +110,11 @@ def test_timesteps_fast_edge_cases(hhs, oracle, torch_cuda):
     assert np.max(np.abs(got[ok] / ref[ok] - 1.0)) <= 1e-14
 
 
-def test_drift_bit_exact(hhs, oracle, torch_cuda):
+@pytest.mark.parametrize("n", [1, 100003, 100004])
+def test_drift_bit_exact(hhs, oracle, torch_cuda, n):
+    """Vector path (16-byte loads) with an odd tail of components (3n odd),
+    and an even count."""
     torch = torch_cuda
-    n = 100003
     rng = np.random.default_rng(0)
     pos, vel = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
     p = torch.as_tensor(pos).cuda()
@@ -120,6 +122,33 @@ def test_drift_bit_exact(hhs, oracle, torch_cuda):
     hhs.drift(p, v, dt=0.0123, fac=0.5)
     torch.cuda.synchronize()
     assert np.array_equal(p.cpu().numpy(), oracle.drift(pos, vel, 0.0123, 0.5))
+
+
+def test_drift_unaligned_and_fp32(hhs, oracle, torch_cuda):
+    """Arrays that start 8 bytes past a 16-byte boundary take the scalar
+    path; FP32 arrays the four-component vector path with a tail.  Both give
+    the element expression pos + vel * h (FP64 bit-exact; FP32 against the
+    same expression in FP32)."""
+    torch = torch_cuda
+    n = 5001
+    rng = np.random.default_rng(3)
+    pos, vel = rng.standard_normal((n + 1, 3)), rng.standard_normal((n + 1, 3))
+    p = torch.as_tensor(pos).cuda().reshape(-1)
+    v = torch.as_tensor(vel).cuda().reshape(-1)
+    ps, vs = p[1:1 + 3 * n], v[1:1 + 3 * n]   # 8 bytes past the allocation's start
+    hhs.drift(ps, vs, n=n, dt=0.0123, fac=0.5)
+    torch.cuda.synchronize()
+    ref = pos.reshape(-1).copy()
+    ref[1:1 + 3 * n] = oracle.drift(pos.reshape(-1)[1:1 + 3 * n].reshape(n, 3),
+                                    vel.reshape(-1)[1:1 + 3 * n].reshape(n, 3), 0.0123,
+                                    0.5).reshape(-1)
+    assert np.array_equal(p.cpu().numpy(), ref)
+    p32, v32 = pos[:n].astype(np.float32), vel[:n].astype(np.float32)
+    d32 = torch.as_tensor(p32).cuda()
+    hhs.drift(d32, torch.as_tensor(v32).cuda(), dt=0.0123, fac=0.5)
+    torch.cuda.synchronize()
+    h = np.float32(0.0123) * np.float32(0.5)
+    assert np.array_equal(d32.cpu().numpy(), p32 + v32 * h)
 
 
 def test_update_inverse_permutation(hhs, oracle, torch_cuda):
