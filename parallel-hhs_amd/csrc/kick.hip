@@ -445,23 +445,36 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
 
 // Split-K for small sink counts: more workgroups over chunks of the same
 // source list while the grid stays under 8192 waves (8 per SIMD on 256
-// CUs), each wave keeps >= 32 sources and the FP64 partials (24 B per sink
-// and chunk) stay within kMaxPartialBytes.  At N = 4096 a plain launch is
+// CUs; see below for large groups), each wave keeps enough sources and the
+// FP64 partials (24 B per sink and chunk) stay within kMaxPartialBytes.  At N = 4096 a plain launch is
 // 512 waves, each running 512 sources back to back with nothing to hide
 // its latency.  MI355X, square kicks: N = 4096 48 -> 21 us, 8192 96 ->
 // 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.  From 512 sources up
 // (1024 before): the HOLD calls of a few hundred to a thousand particles
 // gain more than the reduce launch costs (N = 1024 to t = 1 0.375 ->
 // 0.361 s, N = 16384 in one root step 0.415 -> 0.402 s; 768 fell between).
+// Measured on whole HOLD runs, a sink group of at least 2^28 interactions
+// (the skinny kicks of the upper levels of a large system: a few thousand
+// slow sinks over every source) may fill up to 32768 waves, four rounds of
+// the chip, and every wave keeps >= 16 sources (32 before): N = 131072 root
+// step 1.436 -> 1.378-1.385 s, N = 16384 HOLD 0.361 -> 0.353-0.354 s,
+// N = 4096 to t = 0.25 0.237 -> 0.232-0.235 s, N = 1024 to t = 1 0.357 ->
+// 0.350-0.354 s.  A 16384-wave cap for every group, or >= 8 sources per
+// wave, did worse.
 constexpr size_t kMaxPartialBytes = size_t{4} << 20;
 constexpr int kSplitMinSources = 512;
+constexpr long long kSplitWaves = 8192, kSplitWavesBig = 32768;
+constexpr double kSplitBigWork = 268435456.0;  // 2^28 interactions
+constexpr int kSplitMinChunk = 16;             // sources per wave
 
 int pick_split(int n_sink, int n_src, int S, int W) {
     const long long tiles = (n_sink + 64LL * S - 1) / (64LL * S);
     int K = 1;
     if (n_sink <= 0 || n_src < kSplitMinSources)  // the reduce launch would cost more
         return K;
-    while (K < 64 && tiles * W * K * 2 <= 8192 && n_src / (W * K * 2) >= 32 &&
+    const long long cap = static_cast<double>(n_sink) * n_src >= kSplitBigWork ? kSplitWavesBig
+                                                                                : kSplitWaves;
+    while (K < 64 && tiles * W * K * 2 <= cap && n_src / (W * K * 2) >= kSplitMinChunk &&
            24ULL * n_sink * K * 2 <= kMaxPartialBytes)
         K *= 2;
     return K;
