@@ -137,8 +137,12 @@ int pick_waves(int n) {
 // while the grid stays within kTsMaxWaves waves, each wave keeps >=
 // kTsMinChunk sources and the partials (two values per sink and chunk) stay
 // within 4 MB.  The limits follow the autotuner's choices on MI355X (FP32:
-// n = 8192 K 8 -> 16, 16384 4 -> 8, 32768 2 -> 4, 15-20 % faster).
-constexpr int kTsMinN = 256, kTsMinChunk = 16;
+// n = 8192 K 8 -> 16, 16384 4 -> 8, 32768 2 -> 4, 15-20 % faster).  The
+// chunk floor of 8 sources (16 before) comes from whole HOLD runs, where
+// the calls of a few hundred to a few thousand particles are latency-bound:
+// N = 1024 to t = 1 0.351 -> 0.345 s, N = 4096 to t = 0.25 0.234 -> 0.229 s,
+// N = 16384 in one root step 0.354 -> 0.349 s (4 sources: no better).
+constexpr int kTsMinN = 256, kTsMinChunk = 8;
 constexpr long long kTsMaxWaves = 16384;
 template <typename V> int pick_ts_split(int n, int W) {
     const long long tiles = (n + 63) / 64;
