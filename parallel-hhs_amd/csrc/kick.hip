@@ -425,13 +425,15 @@ int size_class(int n) { return n <= 1 ? 0 : 31 - __builtin_clz(static_cast<unsig
 // W = 4 2377, S = 4 / W = 4 2150; at N = 16384, S = 1 / W = 8 1413 and
 // S = 2 / W = 8 895.  Latency is hidden by waves, not by deeper register
 // blocking: one sink per lane and up to 8 waves per workgroup while each
-// wave keeps >= 64 sources.
+// wave keeps >= 32 sources (64 before: on whole HOLD runs with the split
+// below from 256 sources, N = 1024 to t = 1 0.344 -> 0.316 s, N = 4096 to
+// t = 0.25 0.232 -> 0.207 s; 16 did no better, 128 worse).
 KickConfig kick_pick_config(int n_sink, int n_src) {
     KickConfig c;
     const int fs = g_force_s.load(), fw = g_force_w.load();
     c.sinks_per_lane = 1;
     int w = 1;
-    while (w < 8 && n_src / (2 * w) >= 64) w *= 2;
+    while (w < 8 && n_src / (2 * w) >= 32) w *= 2;
     c.waves = w;
     const int tuned = g_tuned[size_class(n_sink)].load();
     if (tuned) {
@@ -449,10 +451,11 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
 // FP64 partials (24 B per sink and chunk) stay within kMaxPartialBytes.  At N = 4096 a plain launch is
 // 512 waves, each running 512 sources back to back with nothing to hide
 // its latency.  MI355X, square kicks: N = 4096 48 -> 21 us, 8192 96 ->
-// 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.  From 512 sources up
-// (1024 before): the HOLD calls of a few hundred to a thousand particles
-// gain more than the reduce launch costs (N = 1024 to t = 1 0.375 ->
-// 0.361 s, N = 16384 in one root step 0.415 -> 0.402 s; 768 fell between).
+// 39 us, 16384 194 -> 122 us, 32768 570 -> 460 us.  From 256 sources up
+// (1024, then 512 before): the HOLD calls of a few hundred to a thousand
+// particles gain more than the reduce launch costs (1024 -> 512: N = 1024
+// to t = 1 0.375 -> 0.361 s, N = 16384 in one root step 0.415 -> 0.402 s;
+// 512 -> 256 together with the wave rule above: see there).
 // Measured on whole HOLD runs, a sink group of at least 2^28 interactions
 // (the skinny kicks of the upper levels of a large system: a few thousand
 // slow sinks over every source) may fill up to 32768 waves, four rounds of
@@ -462,7 +465,7 @@ KickConfig kick_pick_config(int n_sink, int n_src) {
 // 0.350-0.354 s.  A 16384-wave cap for every group, or >= 8 sources per
 // wave, did worse.
 constexpr size_t kMaxPartialBytes = size_t{4} << 20;
-constexpr int kSplitMinSources = 512;
+constexpr int kSplitMinSources = 256;
 constexpr long long kSplitWaves = 8192, kSplitWavesBig = 32768;
 constexpr double kSplitBigWork = 268435456.0;  // 2^28 interactions
 constexpr int kSplitMinChunk = 16;             // sources per wave
