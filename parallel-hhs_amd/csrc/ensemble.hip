@@ -1350,7 +1350,13 @@ __global__ __launch_bounds__(256) void rank_items_kernel(EnsArgs a) {
 // (both team sizes), which the register allocator needs to stay within
 // that budget without spilling.
 template <bool SOFT, bool GUARD>
-__global__ __launch_bounds__(kWG, 3) __attribute__((flatten)) void ensemble_kernel(EnsArgs a) {
+// Four waves per SIMD (<= 128 VGPRs, four workgroups per CU; the LDS of
+// four, 4 x 40 KB, still fits): the compiler then spills ~200 bytes per lane
+// to scratch, but the batch runs faster than at three waves per SIMD with
+// no spills (MI355X, 4096-system workload: 5.24-5.55 s against 6.39-6.47 s,
+// same bits).  Fewer sinks per lane to avoid the spills lost more than the
+// occupancy gained (two: 6.8 s, one: 7.3 s).
+__global__ __launch_bounds__(kWG, 4) __attribute__((flatten)) void ensemble_kernel(EnsArgs a) {
     __shared__ TeamShared<kWG> sh_wg;
     __shared__ TeamShared<64> sh[kWaves];
     // Source tiles: the whole array for a workgroup team (kWG sources), a
@@ -1666,14 +1672,39 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     a.ts_mode = d->timestep_mode;
     set_guard(a, d->approximate_decisions ? HHS_TIMESTEP_EXACT : a.ts_mode);
 
-    // Persistent grid: enough workgroups for every big system and every
-    // group of four small ones, capped at 8 per CU.
+    // compute units of the device (the persistent grid's cap below)
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             v > 0)
             cus = v;
+    }
+    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula; GUARD:
+    // the decision guard (approximate timestep modes, unless the descriptor
+    // asks for approximate decisions).  The guard is a template parameter:
+    // compiled into the flattened engine behind a run-time test it slowed
+    // every mode down (MI355X, 4096-system batch: 7.7-8.0 s against 6.1-6.4
+    // s, also in the exact mode where it never runs); as a separate
+    // instantiation both run at 6.2-6.6 s.
+    auto kern = a.eps2f > 0.0f ? (a.guard ? ensemble_kernel<true, true> : ensemble_kernel<true, false>)
+                               : (a.guard ? ensemble_kernel<false, true> : ensemble_kernel<false, false>);
+    // Persistent grid: enough workgroups for every big system and every
+    // group of four small ones, capped at 8 per CU.
+    const int grid = std::max(1, std::min(a.n_items, 8 * cus));
+    // Warm-up: the kernel spills to scratch, and the first dispatch that
+    // needs scratch on a hardware queue ran the whole batch slowly (7.3-8.7
+    // s against 5.2-5.3 s for later ones on the same queue).  The same grid
+    // with no items (every workgroup takes a ticket >= 0 = n_items and
+    // leaves) takes that first dispatch on every launch, for a few
+    // microseconds; the tickets it drew are cleared again.
+    {
+        EnsArgs z = a;
+        z.n_items = 0;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWG), 0, st, z);
+        HHS_LAUNCH_CHECK("ensemble_kernel (warm-up)");
+        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(256), 0, st, counters);
+        HHS_LAUNCH_CHECK("zero_counters_kernel");
     }
     // Cost estimates, then the work queue in estimate order.
     hipLaunchKernelGGL(ensemble_estimate_kernel, dim3(nsys), dim3(64), 0, st, a);
@@ -1684,16 +1715,6 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     }
     hipLaunchKernelGGL(rank_items_kernel, dim3((a.n_items + 255) / 256), dim3(256), 0, st, a);
     HHS_LAUNCH_CHECK("rank_items_kernel");
-    const int grid = std::max(1, std::min(a.n_items, 8 * cus));
-    // SOFT: eps^2 > 0 in FP32, so r^2 > 0 in every FP32 pair formula; GUARD:
-    // the decision guard (approximate timestep modes, unless the descriptor
-    // asks for approximate decisions).  The guard is a template parameter:
-    // compiled into the flattened engine behind a run-time test it slowed
-    // every mode down (MI355X, 4096-system batch: 7.7-8.0 s against 6.1-6.4
-    // s, also in the exact mode where it never runs); as a separate
-    // instantiation both run at 6.2-6.6 s.
-    auto kern = a.eps2f > 0.0f ? (a.guard ? ensemble_kernel<true, true> : ensemble_kernel<true, false>)
-                               : (a.guard ? ensemble_kernel<false, true> : ensemble_kernel<false, false>);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWG), 0, st, a);
     HHS_LAUNCH_CHECK("ensemble_kernel");
     return HHS_OK;
