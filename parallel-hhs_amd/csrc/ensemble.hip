@@ -26,7 +26,10 @@
 //   FP64-derived FP32 separations with two-sum compensated FP32 force sums,
 //   bit-exact FP64 pair timesteps (pair_math.h).
 #include <algorithm>
+#include <mutex>
 #include <numeric>
+#include <set>
+#include <tuple>
 #include <vector>
 
 #include "hhs_internal.h"
@@ -1268,6 +1271,11 @@ __device__ void team_run_system(const EnsArgs &a, int s, TeamShared<TEAM> &sh, d
 // level L or deeper, a root step makes about 2^L calls on n_L particles,
 // each costing ~n_L^2 pair evaluations plus a fixed overhead.
 constexpr double kCallCost = 16384.0;  // per-call overhead, in pair evaluations
+
+// (device, stream, kernel) triples whose first ensemble launch ran the
+// scratch warm-up (see hhs_ensemble_run_f64)
+std::mutex g_warm_mu;
+std::set<std::tuple<int, hipStream_t, const void *>> g_warmed;
 constexpr int kEstSinks = 1024;        // sinks sampled per system
 
 __global__ __launch_bounds__(64) void ensemble_estimate_kernel(EnsArgs a) {
@@ -1694,14 +1702,23 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     const int grid = std::max(1, std::min(a.n_items, 8 * cus));
     // Warm-up: the kernel spills to scratch, and the first dispatch that
     // needs scratch on a hardware queue ran the whole batch slowly (7.3-8.7
-    // s against 5.2-5.3 s for later ones on the same queue).  The same grid
-    // with no items (every workgroup takes a ticket >= 0 = n_items and
-    // leaves) takes that first dispatch on every launch, for a few
-    // microseconds; the tickets it drew are cleared again.
+    // s against 5.2-5.3 s for later ones on the same queue; a one-workgroup
+    // warm-up left 5.7-6.7 s).  So the first launch of each kernel variant
+    // on a stream is preceded by the largest grid with no items: every
+    // workgroup takes a ticket >= 0 = n_items and leaves, within
+    // microseconds, and the tickets it drew are cleared again.  Streams
+    // share a few hardware queues, so some warm-ups are redundant; a stream
+    // created at a freed stream's address on a cold queue only runs its
+    // first batch slowly.
+    bool warm = false;
     {
+        std::lock_guard<std::mutex> lk(g_warm_mu);
+        warm = g_warmed.insert({dev, st, reinterpret_cast<const void *>(kern)}).second;
+    }
+    if (warm) {
         EnsArgs z = a;
         z.n_items = 0;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWG), 0, st, z);
+        hipLaunchKernelGGL(kern, dim3(8 * cus), dim3(kWG), 0, st, z);
         HHS_LAUNCH_CHECK("ensemble_kernel (warm-up)");
         hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(256), 0, st, counters);
         HHS_LAUNCH_CHECK("zero_counters_kernel");

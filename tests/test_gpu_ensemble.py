@@ -148,3 +148,24 @@ def test_ensemble_team_size_does_not_change_bits(hhs, torch_cuda):
     assert ra.tobytes() == rb.tobytes()
     for (pa, va), (pb, vb) in zip(sa, sb):
         assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+
+
+def test_ensemble_same_bits_on_new_streams(hhs, torch_cuda):
+    """The first launch of each kernel variant on a stream runs a warm-up
+    dispatch (no items) before the batch; the batch's bits must not depend
+    on it: the same batch on the current stream and on fresh streams, in
+    the FP32 and the exact timestep modes."""
+    eps, eta, t_end, dtr = 1e-3, 0.01, 0.125, 1.0 / 16
+    sizes = [9, 40, 130, 513, 700]
+    systems = _systems(hhs, sizes, 1300, eps)
+    for mode in (hhs.TIMESTEP_FP32, hhs.TIMESTEP_EXACT):
+        runs = []
+        for stream in (None, torch_cuda.cuda.Stream(), torch_cuda.cuda.Stream(), None):
+            ens = hhs.Ensemble(systems, t_end, dtr, eta, eps, fast_timesteps=mode)
+            ens.run(stream)
+            torch_cuda.cuda.synchronize()
+            assert (ens.results()["status"] == 0).all()
+            runs.append([ens.state(k) for k in range(len(sizes))])
+        for other in runs[1:]:
+            for (pa, va), (pb, vb) in zip(runs[0], other):
+                assert pa.tobytes() == pb.tobytes() and va.tobytes() == vb.tobytes()
