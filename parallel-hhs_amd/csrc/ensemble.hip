@@ -1713,6 +1713,7 @@ int hhs_ensemble_run_f64(const hhs_ensemble_desc *d, const int *offsets_host, vo
     bool warm = false;
     {
         std::lock_guard<std::mutex> lk(g_warm_mu);
+        if (g_warmed.size() >= 4096) g_warmed.clear();  // streams come and go: bound the record
         warm = g_warmed.insert({dev, st, reinterpret_cast<const void *>(kern)}).second;
     }
     if (warm) {
