@@ -1441,11 +1441,7 @@ __global__ __launch_bounds__(LDS ? kSubTeam : kWG) __attribute__((flatten)) void
         // kick_reduce_kernel would have applied it
         double kv[3] = {0.0, 0.0, 0.0};
         if (kp.part) {
-            for (int k = 0; k < kp.K; ++k) {
-                const double *q = kp.part + 3 * (static_cast<size_t>(k) * kp.rows + i);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) kv[c] += q[c];
-            }
+            sum_kick_partials(kp.part, kp.K, kp.rows, i, kv[0], kv[1], kv[2]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) kv[c] = __dmul_rn(kv[c], kp.h);
         }

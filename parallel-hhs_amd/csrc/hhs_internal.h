@@ -116,6 +116,40 @@ struct KickPending {
     int rows = 0;
     double h = 0.0;
 };
+
+// Row i's sum over the K chunk partials part[k][rows][3], added in chunk
+// order from 0.0 (the split-K reduce's arithmetic, so the same bits as a
+// plain loop).  Eight chunks' loads are issued before their adds: the plain
+// loop waited for each chunk's load in turn, one memory round trip per
+// chunk.  K is uniform over the wave.
+__device__ __forceinline__ void sum_kick_partials(const double *__restrict__ part, int K,
+                                                  int rows, int i, double &ax, double &ay,
+                                                  double &az) {
+    constexpr int B = 8;
+    ax = 0.0;
+    ay = 0.0;
+    az = 0.0;
+    for (int k = 0; k < K; k += B) {
+        double v[B][3];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (k + j < K) {
+                const double *q = part + 3 * (static_cast<size_t>(k + j) * rows + i);
+                v[j][0] = q[0];
+                v[j][1] = q[1];
+                v[j][2] = q[2];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (k + j < K) {
+                ax += v[j][0];
+                ay += v[j][1];
+                az += v[j][2];
+            }
+        }
+    }
+}
 // defer non-null: a split-K launch leaves its reduce to the caller, which
 // gets each sink group's pending partials (a: the first group, b: the
 // second) and must have them applied before anything reads those

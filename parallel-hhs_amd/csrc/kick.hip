@@ -334,13 +334,8 @@ __global__ __launch_bounds__(256) void kick_reduce_kernel(T *__restrict__ ga_vel
     const int rows = grp_b ? n_sink_b : n_sink;
     const int K = grp_b ? K_b : K_a;
     const double *base = partial + (grp_b ? 3 * static_cast<size_t>(K_a > 1 ? K_a : 0) * n_sink : 0);
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int k = 0; k < K; ++k) {
-        const double *q = base + 3 * (static_cast<size_t>(k) * rows + i);
-        ax += q[0];
-        ay += q[1];
-        az += q[2];
-    }
+    double ax, ay, az;
+    sum_kick_partials(base, K, rows, i, ax, ay, az);
     T *si_vel = grp_b ? gb_vel : ga_vel;
     T *si_acc = grp_b ? gb_acc : ga_acc;
     const T a0 = static_cast<T>(ax), a1 = static_cast<T>(ay), a2 = static_cast<T>(az);

@@ -98,13 +98,8 @@ __global__ __launch_bounds__(256) void update_pair_kernel(Particles<T> prev, Par
     T vx = c.vel[3 * i + 0], vy = c.vel[3 * i + 1], vz = c.vel[3 * i + 2];
     const T m = c.m[i], ts = c.timestep[i];
     if (!is_fast && kp.part) {
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        for (int k = 0; k < kp.K; ++k) {
-            const double *q = kp.part + 3 * (static_cast<size_t>(k) * kp.rows + i);
-            ax += q[0];
-            ay += q[1];
-            az += q[2];
-        }
+        double ax, ay, az;
+        sum_kick_partials(kp.part, kp.K, kp.rows, i, ax, ay, az);
         const T hk = static_cast<T>(kp.h);
         vx = vx + static_cast<T>(ax) * hk;
         vy = vy + static_cast<T>(ay) * hk;
@@ -152,13 +147,8 @@ __global__ __launch_bounds__(256) void update_chain_kernel(Particles<T> prev, Sc
     if (slow) {
         const KickPending kp = c.lv[k].kick;
         if (kp.part) {
-            double ax = 0.0, ay = 0.0, az = 0.0;
-            for (int q = 0; q < kp.K; ++q) {
-                const double *r = kp.part + 3 * (static_cast<size_t>(q) * kp.rows + i);
-                ax += r[0];
-                ay += r[1];
-                az += r[2];
-            }
+            double ax, ay, az;
+            sum_kick_partials(kp.part, kp.K, kp.rows, i, ax, ay, az);
             const T hk = static_cast<T>(kp.h);
             vx = vx + static_cast<T>(ax) * hk;
             vy = vy + static_cast<T>(ay) * hk;

@@ -117,9 +117,23 @@ __global__ __launch_bounds__(256) void timesteps_merge_kernel(
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     TsAcc<T, MODE> acc;
-    for (int k = 0; k < K; ++k) {
-        const auto *q = partial + 2 * (static_cast<size_t>(k) * n + i);
-        acc.merge(q[0], q[1]);
+    // eight chunks' loads issued before their folds (in chunk order, as a
+    // plain loop would fold them): one memory round trip per eight chunks
+    constexpr int B = 8;
+    using V = typename TsAcc<T, MODE>::V;
+    for (int k = 0; k < K; k += B) {
+        V num[B], den[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (k + j < K) {
+                const V *q = partial + 2 * (static_cast<size_t>(k + j) * n + i);
+                num[j] = q[0];
+                den[j] = q[1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+            if (k + j < K) acc.merge(num[j], den[j]);
     }
     ts[i] = acc.value();
 }
